@@ -1,0 +1,114 @@
+"""Recipe that compiles the Fortran reference where it lies and runs it
+(TEST INFRASTRUCTURE ONLY; needs /root/reference, so it runs only in the
+build container).
+
+The reference sizes its arrays from an INCLUDEd PARAMETER block, so there is
+one build per box.  For each box a directory oracle/_ref/<name>/ receives the
+generated 3dmhdparam.f / 3dmhdset.f and *symlinks* to the reference sources;
+Fortran INCLUDE resolves next to the (symlinked) source, so the reference
+compiles against the generated parameter block without its source being
+copied.  The symlinks are removed once the objects exist, so oracle/_ref
+holds only generated files, objects, executables and run output.
+
+Toolchain: amdflang (ROCm flang) with the conda MPICH Fortran bindings,
+-O2 on baseline x86-64 with FP contraction off.
+"""
+
+from __future__ import annotations
+
+import importlib.util
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+REF_SRC = "/root/reference"
+REF_OUT = os.path.join(HERE, "_ref")
+MPI_INC = "/opt/conda/include"
+MPI_LIB = "/opt/conda/lib"
+MPIEXEC = "/opt/conda/bin/mpiexec"
+FC = "amdflang"
+FFLAGS = ["-O2", "-ffixed-form", "-ffp-contract=off", f"-I{MPI_INC}"]
+
+
+def _load(path, name):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+mkparam = _load(os.path.join(REPO, "rast-3dmhd_amd", "fortran", "mkparam.py"), "mkparam")
+refconfigs = _load(os.path.join(HERE, "refconfigs.py"), "refconfigs")
+
+
+def available() -> bool:
+    return os.path.isfile(os.path.join(REF_SRC, "3dmhdsub.f")) and shutil.which(FC) is not None
+
+
+def _run(cmd, cwd):
+    r = subprocess.run(cmd, cwd=cwd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd)} failed in {cwd}:\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build(name: str, cfg: dict | None = None, main_exe: bool = False) -> str:
+    """Compile the reference for one box; returns the build directory."""
+    cfg = cfg or refconfigs.CONFIGS[name]
+    d = os.path.join(REF_OUT, name)
+    os.makedirs(d, exist_ok=True)
+    mkparam.write_config(d, cfg["params"], par=cfg["par"], ipar=cfg.get("ipar"))
+    links = {"3dmhdsub.f": os.path.join(REF_SRC, "3dmhdsub.f"),
+             "refharness.f": os.path.join(HERE, "refharness.f")}
+    if main_exe:
+        links["3dmhd.f"] = os.path.join(REF_SRC, "3dmhd.f")
+    try:
+        for ln, target in links.items():
+            p = os.path.join(d, ln)
+            if os.path.lexists(p):
+                os.remove(p)
+            os.symlink(target, p)
+        for src in links:
+            _run([FC, *FFLAGS, "-c", src, "-o", src[:-2] + ".o"], d)
+        _run([FC, *FFLAGS, "-c", "3dmhdset.f", "-o", "3dmhdset.o"], d)
+    finally:
+        for ln in links:
+            p = os.path.join(d, ln)
+            if os.path.islink(p):
+                os.remove(p)
+    libs = [f"-L{MPI_LIB}", f"-Wl,-rpath,{MPI_LIB}", "-lmpifort", "-lmpi"]
+    _run([FC, "refharness.o", "3dmhdsub.o", "3dmhdset.o", *libs, "-o", "harness.exe"], d)
+    if main_exe:
+        _run([FC, "3dmhd.o", "3dmhdsub.o", "3dmhdset.o", *libs, "-o", "3dmhd.exe"], d)
+    return d
+
+
+def run_harness(name: str, cfg: dict | None = None) -> str:
+    """Run the harness for one box under MPI; returns the output directory."""
+    cfg = cfg or refconfigs.CONFIGS[name]
+    d = os.path.join(REF_OUT, name)
+    out = os.path.join(d, "run")
+    if os.path.isdir(out):
+        shutil.rmtree(out)
+    os.makedirs(out)
+    dumps = cfg["dumps"]
+    with open(os.path.join(out, "harness.in"), "w") as f:
+        f.write(f"{cfg['steps']}\n")
+        f.write(f"{len(dumps)} " + " ".join(str(x) for x in dumps) + "\n")
+        f.write(f"{cfg['ampu']} {cfg['ampb']} {cfg['seed']}\n")
+    p = cfg["params"]
+    npe = int(p.get("NPEY", 1)) * int(p.get("NPEZ", 2))
+    _run([MPIEXEC, "-launcher", "fork", "-n", str(npe), os.path.join(d, "harness.exe")], out)
+    return out
+
+
+if __name__ == "__main__":
+    if not available():
+        sys.exit("reference sources or amdflang not available")
+    names = sys.argv[1:] or list(refconfigs.CONFIGS)
+    for n in names:
+        print("building", n, build(n), flush=True)
+        print("running", n, run_harness(n), flush=True)

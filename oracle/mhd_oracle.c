@@ -1,0 +1,803 @@
+/*
+ * CPU oracle for the explicit compressible-MHD time step of Rast's 3dmhd code.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C FP64 restatement of the
+ * reference's STEP / FLUXES / BCON / COMM_MPI (3dmhdsub.f) for a single
+ * resident domain.  Only tests/, the smoke entry point and bench.py's CPU
+ * baseline leg may load it, and only as the checker.  The product path is the
+ * HIP library under rast-3dmhd_amd/csrc and never calls into this file.
+ *
+ * Layout: every 3-D array is the reference's padded Fortran-order array
+ * (NX,NY,NZ) = (NPX+2, NPY+2, NPZ+2), x fastest, with one ghost layer per
+ * side.  Indices below are 1-based like the reference (I=1..NX etc.).
+ *
+ * The restatement follows the reference loop by loop, keeping the scratch
+ * arrays and the left-to-right evaluation order of every expression, so that
+ * it is bit-for-bit equal to the Fortran build when compiled without FMA
+ * contraction (-ffp-contract=off).  The pinning evidence is in
+ * tests/test_oracle_golden.py (fixtures written by the reference itself).
+ *
+ * Single-domain semantics: where the reference branches on MYPEZ.EQ.0 (top)
+ * or MYPEZ.EQ.NPEZ-1 (bottom), this domain is both, so both branches apply.
+ * The top/bottom z ghost planes are never refreshed (COMM_MPI has no
+ * neighbour there), exactly as in the reference.
+ *
+ * Out of scope (rejected by orc_check): density diffusion ID!=0, LREM,
+ * flux relaxation RLAX>0, ITC=2/3/4 and plume boundary TP!=0, LSHR,
+ * non-periodic horizontal boundaries.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mhd_oracle.h"
+
+#define IDX(i, j, k) \
+    ((size_t)((i) - 1) + (size_t)nx * ((size_t)((j) - 1) + (size_t)ny * (size_t)((k) - 1)))
+#define A(p, i, j, k) (p)[IDX(i, j, k)]
+
+/* Metric arrays are 1-based in the reference; here they are 0-based C arrays
+ * of length NX / NY / NZ, so element I of the reference is m[I-1]. */
+#define MX(p, i) (p)[(i) - 1]
+
+int orc_check(const orc_state *s)
+{
+    if (s->nx < 5 || s->ny < 5 || s->nz < 6) return 1; /* one-sided BX stencils reach K+3 */
+    if (s->izc != 0 && s->izc != 1) return 2;          /* BCON: 'Invalid IZCON' (sub:2942) */
+    if (s->itc != 0 && s->itc != 1) return 3;          /* ITC 2/3/4 out of scope */
+    if (s->tp != 0.0) return 4;                        /* plume boundary out of scope */
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * FLUXES (3dmhdsub.f:1178-2637), main path (ID=0, no LREM/RLAX/LSHR).
+ * Reads RU,RV,RW,RO,TT[,BX,BY,BZ]; writes FU,FV,FW,FR,FT and, with LMAG,
+ * WW1,WW2,WW3 = dB/dt.  As in the reference, RO holds 1/rho on return and
+ * RU,RV,RW,TT hold MHD scratch in the interior when LMAG is on.
+ * ------------------------------------------------------------------------- */
+void orc_fluxes(orc_state *s)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    double *RU = s->ru, *RV = s->rv, *RW = s->rw, *RO = s->ro, *TT = s->tt;
+    double *UU = s->uu, *VV = s->vv, *WW = s->ww;
+    double *FU = s->fu, *FV = s->fv, *FW = s->fw, *FR = s->fr, *FT = s->ft;
+    double *W1 = s->ww1, *W2 = s->ww2, *W3 = s->ww3;
+    double *BX = s->bx, *BY = s->by, *BZ = s->bz;
+    const double *DXXDX = s->dxxdx, *D2XXDX2 = s->d2xxdx2;
+    const double *DYYDY = s->dyydy, *D2YYDY2 = s->d2yydy2;
+    const double *DZZDZ = s->dzzdz, *D2ZZDZ2 = s->d2zzdz2;
+    const double *RKAPA = s->rkapa, *DKAPA = s->dkapa;
+    const double HX = s->hx, H2X = s->h2x, HY = s->hy, H2Y = s->h2y, HZ = s->hz, H2Z = s->h2z;
+    const double C13 = s->c13, C43 = s->c43;
+    const double OCV = s->ocv, ORE = s->ore, REPR = s->repr, GRAV = s->grav;
+    const double OMX = s->omx, OMZ = s->omz, ORM = s->orm, OBETA = s->obeta;
+    int i, j, k;
+    double tmp, tmpy, tmpz, tmpy1, tmpy2, tmpz1, tmpz2;
+
+    /* continuity, x and y parts (sub:1252-1261) */
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy = HY * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                A(FR, i, j, k) = (A(RU, i - 1, j, k) - A(RU, i + 1, j, k)) * HX * MX(DXXDX, i);
+                A(FR, i, j, k) = A(FR, i, j, k) - (A(RV, i, j + 1, k) - A(RV, i, j - 1, k)) * tmpy;
+            }
+        }
+    /* d(rho w)/dz, centred then one-sided on the boundary planes (sub:1268-1289) */
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz = HZ * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W1, i, j, k) = (A(RW, i, j, k + 1) - A(RW, i, j, k - 1)) * tmpz;
+    }
+    tmpz = HZ * MX(DZZDZ, 2);
+    for (j = 2; j <= ny - 1; j++)
+        for (i = 2; i <= nx - 1; i++)
+            A(W1, i, j, 2) = (4.0 * A(RW, i, j, 3) - A(RW, i, j, 4)) * tmpz;
+    tmpz = HZ * MX(DZZDZ, nz - 1);
+    for (j = 2; j <= ny - 1; j++)
+        for (i = 2; i <= nx - 1; i++)
+            A(W1, i, j, nz - 1) = (A(RW, i, j, nz - 3) - 4.0 * A(RW, i, j, nz - 2)) * tmpz;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FR, i, j, k) = A(FR, i, j, k) - A(W1, i, j, k);
+
+    /* buoyancy (sub:1376-1380) */
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FW, i, j, k) = GRAV * A(RO, i, j, k);
+
+    /* pressure and in-place 1/rho over every point, ghosts included (sub:1386-1391) */
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                A(W1, i, j, k) = A(RO, i, j, k) * A(TT, i, j, k);
+                A(RO, i, j, k) = 1.0 / A(RO, i, j, k);
+            }
+
+    /* pressure gradient (sub:1396-1404) */
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz = HZ * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy = HY * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                A(FW, i, j, k) = A(FW, i, j, k) - (A(W1, i, j, k + 1) - A(W1, i, j, k - 1)) * tmpz;
+                A(FV, i, j, k) = (A(W1, i, j - 1, k) - A(W1, i, j + 1, k)) * tmpy;
+                A(FU, i, j, k) = (A(W1, i - 1, j, k) - A(W1, i + 1, j, k)) * HX * MX(DXXDX, i);
+            }
+        }
+    }
+
+    /* velocities over every point (sub:1411-1425) */
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                A(UU, i, j, k) = A(RU, i, j, k) * A(RO, i, j, k);
+                A(VV, i, j, k) = A(RV, i, j, k) * A(RO, i, j, k);
+                A(WW, i, j, k) = A(RW, i, j, k) * A(RO, i, j, k);
+            }
+
+    /* momentum advection, conservative centred form (sub:1435-1482) */
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz = HZ * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy = HY * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                A(FU, i, j, k) = A(FU, i, j, k)
+                    - (A(RU, i + 1, j, k) * A(UU, i + 1, j, k) - A(RU, i - 1, j, k) * A(UU, i - 1, j, k)) * HX * MX(DXXDX, i);
+                A(FU, i, j, k) = A(FU, i, j, k)
+                    - (A(RU, i, j + 1, k) * A(VV, i, j + 1, k) - A(RU, i, j - 1, k) * A(VV, i, j - 1, k)) * tmpy;
+                A(FU, i, j, k) = A(FU, i, j, k)
+                    - (A(RU, i, j, k + 1) * A(WW, i, j, k + 1) - A(RU, i, j, k - 1) * A(WW, i, j, k - 1)) * tmpz;
+            }
+        }
+    }
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz = HZ * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy = HY * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                A(FV, i, j, k) = A(FV, i, j, k)
+                    - (A(RV, i + 1, j, k) * A(UU, i + 1, j, k) - A(RV, i - 1, j, k) * A(UU, i - 1, j, k)) * HX * MX(DXXDX, i);
+                A(FV, i, j, k) = A(FV, i, j, k)
+                    - (A(RV, i, j + 1, k) * A(VV, i, j + 1, k) - A(RV, i, j - 1, k) * A(VV, i, j - 1, k)) * tmpy;
+                A(FV, i, j, k) = A(FV, i, j, k)
+                    - (A(RV, i, j, k + 1) * A(WW, i, j, k + 1) - A(RV, i, j, k - 1) * A(WW, i, j, k - 1)) * tmpz;
+            }
+        }
+    }
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz = HZ * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy = HY * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                A(FW, i, j, k) = A(FW, i, j, k)
+                    - (A(RW, i + 1, j, k) * A(UU, i + 1, j, k) - A(RW, i - 1, j, k) * A(UU, i - 1, j, k)) * HX * MX(DXXDX, i);
+                A(FW, i, j, k) = A(FW, i, j, k)
+                    - (A(RW, i, j + 1, k) * A(VV, i, j + 1, k) - A(RW, i, j - 1, k) * A(VV, i, j - 1, k)) * tmpy;
+                A(FW, i, j, k) = A(FW, i, j, k)
+                    - (A(RW, i, j, k + 1) * A(WW, i, j, k + 1) - A(RW, i, j, k - 1) * A(WW, i, j, k - 1)) * tmpz;
+            }
+        }
+    }
+
+    /* temperature advection (sub:1495-1506); the leading unary minus negates
+     * the whole product, which is bit-identical to negating UU first. */
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz = HZ * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy = HY * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                A(FT, i, j, k) = -(A(UU, i, j, k) * (A(TT, i + 1, j, k) - A(TT, i - 1, j, k)) * HX * MX(DXXDX, i));
+                A(FT, i, j, k) = A(FT, i, j, k) - A(VV, i, j, k) * (A(TT, i, j + 1, k) - A(TT, i, j - 1, k)) * tmpy;
+                A(FT, i, j, k) = A(FT, i, j, k) - A(WW, i, j, k) * (A(TT, i, j, k + 1) - A(TT, i, j, k - 1)) * tmpz;
+            }
+        }
+    }
+
+    /* thermal conduction with depth-dependent kappa (sub:1584-1610) */
+    tmp = OCV / REPR;
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz1 = HZ * MX(D2ZZDZ2, k);
+        tmpz2 = H2Z * MX(DZZDZ, k) * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy1 = HY * MX(D2YYDY2, j);
+            tmpy2 = H2Y * MX(DYYDY, j) * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                const double t0 = A(TT, i, j, k);
+                A(FT, i, j, k) = A(FT, i, j, k)
+                    + ((A(TT, i + 1, j, k) - A(TT, i - 1, j, k)) * HX * MX(D2XXDX2, i)
+                       + (A(TT, i + 1, j, k) - 2.0 * t0 + A(TT, i - 1, j, k)) * H2X * MX(DXXDX, i) * MX(DXXDX, i))
+                      * A(RO, i, j, k) * tmp * MX(RKAPA, k);
+                A(FT, i, j, k) = A(FT, i, j, k)
+                    + ((A(TT, i, j + 1, k) - A(TT, i, j - 1, k)) * tmpy1
+                       + (A(TT, i, j + 1, k) - 2.0 * t0 + A(TT, i, j - 1, k)) * tmpy2)
+                      * A(RO, i, j, k) * tmp * MX(RKAPA, k);
+                A(FT, i, j, k) = A(FT, i, j, k)
+                    + ((A(TT, i, j, k + 1) - A(TT, i, j, k - 1)) * tmpz1
+                       + (A(TT, i, j, k + 1) - 2.0 * t0 + A(TT, i, j, k - 1)) * tmpz2)
+                      * A(RO, i, j, k) * tmp * MX(RKAPA, k);
+                A(FT, i, j, k) = A(FT, i, j, k)
+                    + (A(TT, i, j, k + 1) - A(TT, i, j, k - 1)) * HZ * MX(DZZDZ, k) * A(RO, i, j, k) * tmp * MX(DKAPA, k);
+            }
+        }
+    }
+
+    /* viscous Laplacians with the 4/3 factor on the normal component (sub:1818-1886) */
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz1 = HZ * MX(D2ZZDZ2, k);
+        tmpz2 = H2Z * MX(DZZDZ, k) * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy1 = HY * MX(D2YYDY2, j);
+            tmpy2 = H2Y * MX(DYYDY, j) * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                const double u0 = A(UU, i, j, k);
+                A(FU, i, j, k) = A(FU, i, j, k) + ORE * (
+                    C43 * ((A(UU, i + 1, j, k) - A(UU, i - 1, j, k)) * HX * MX(D2XXDX2, i)
+                           + (A(UU, i + 1, j, k) - 2.0 * u0 + A(UU, i - 1, j, k)) * H2X * MX(DXXDX, i) * MX(DXXDX, i))
+                    + (A(UU, i, j + 1, k) - A(UU, i, j - 1, k)) * tmpy1
+                    + (A(UU, i, j + 1, k) - 2.0 * u0 + A(UU, i, j - 1, k)) * tmpy2
+                    + (A(UU, i, j, k + 1) - A(UU, i, j, k - 1)) * tmpz1
+                    + (A(UU, i, j, k + 1) - 2.0 * u0 + A(UU, i, j, k - 1)) * tmpz2);
+            }
+        }
+    }
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz1 = HZ * MX(D2ZZDZ2, k);
+        tmpz2 = H2Z * MX(DZZDZ, k) * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy1 = HY * MX(D2YYDY2, j);
+            tmpy2 = H2Y * MX(DYYDY, j) * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                const double v0 = A(VV, i, j, k);
+                A(FV, i, j, k) = A(FV, i, j, k) + ORE * (
+                    (A(VV, i + 1, j, k) - A(VV, i - 1, j, k)) * HX * MX(D2XXDX2, i)
+                    + (A(VV, i + 1, j, k) - 2.0 * v0 + A(VV, i - 1, j, k)) * H2X * MX(DXXDX, i) * MX(DXXDX, i)
+                    + C43 * ((A(VV, i, j + 1, k) - A(VV, i, j - 1, k)) * tmpy1
+                             + (A(VV, i, j + 1, k) - 2.0 * v0 + A(VV, i, j - 1, k)) * tmpy2)
+                    + (A(VV, i, j, k + 1) - A(VV, i, j, k - 1)) * tmpz1
+                    + (A(VV, i, j, k + 1) - 2.0 * v0 + A(VV, i, j, k - 1)) * tmpz2);
+            }
+        }
+    }
+    for (k = 2; k <= nz - 1; k++) {
+        tmpz1 = HZ * MX(D2ZZDZ2, k);
+        tmpz2 = H2Z * MX(DZZDZ, k) * MX(DZZDZ, k);
+        for (j = 2; j <= ny - 1; j++) {
+            tmpy1 = HY * MX(D2YYDY2, j);
+            tmpy2 = H2Y * MX(DYYDY, j) * MX(DYYDY, j);
+            for (i = 2; i <= nx - 1; i++) {
+                const double w0 = A(WW, i, j, k);
+                A(FW, i, j, k) = A(FW, i, j, k) + ORE * (
+                    (A(WW, i + 1, j, k) - A(WW, i - 1, j, k)) * HX * MX(D2XXDX2, i)
+                    + (A(WW, i + 1, j, k) - 2.0 * w0 + A(WW, i - 1, j, k)) * H2X * MX(DXXDX, i) * MX(DXXDX, i)
+                    + (A(WW, i, j + 1, k) - A(WW, i, j - 1, k)) * tmpy1
+                    + (A(WW, i, j + 1, k) - 2.0 * w0 + A(WW, i, j - 1, k)) * tmpy2
+                    + C43 * ((A(WW, i, j, k + 1) - A(WW, i, j, k - 1)) * tmpz1
+                             + (A(WW, i, j, k + 1) - 2.0 * w0 + A(WW, i, j, k - 1)) * tmpz2));
+            }
+        }
+    }
+
+    /* viscous heating, compressional heating and the (1/3) grad div u cross
+     * terms, through the WW1..WW3 scratch arrays (sub:1891-2101) */
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W1, i, j, k) = (A(UU, i + 1, j, k) - A(UU, i - 1, j, k)) * HX * MX(DXXDX, i);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W2, i, j, k) = (A(VV, i + 1, j, k) - A(VV, i - 1, j, k)) * HX * MX(DXXDX, i);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W3, i, j, k) = (A(WW, i + 1, j, k) - A(WW, i - 1, j, k)) * HX * MX(DXXDX, i);
+    tmp = OCV * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k)
+                    + (C43 * A(W1, i, j, k) * A(W1, i, j, k) + A(W2, i, j, k) * A(W2, i, j, k)
+                       + A(W3, i, j, k) * A(W3, i, j, k)) * A(RO, i, j, k) * tmp;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k) - OCV * A(W1, i, j, k) * A(TT, i, j, k);
+    tmp = C13 * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FU, i, j, k) = A(FU, i, j, k) + tmp * (A(W2, i, j + 1, k) - A(W2, i, j - 1, k)) * HY * MX(DYYDY, j);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FV, i, j, k) = A(FV, i, j, k) + tmp * (A(W1, i, j + 1, k) - A(W1, i, j - 1, k)) * HY * MX(DYYDY, j);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W1, i, j, k) = (A(UU, i, j + 1, k) - A(UU, i, j - 1, k)) * HY * MX(DYYDY, j);
+    tmp = OCV * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k)
+                    + (A(W1, i, j, k) * A(W1, i, j, k) + 2.0 * A(W1, i, j, k) * A(W2, i, j, k)) * A(RO, i, j, k) * tmp;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W2, i, j, k) = (A(VV, i, j + 1, k) - A(VV, i, j - 1, k)) * HY * MX(DYYDY, j);
+    tmp = C43 * OCV * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k) + A(W2, i, j, k) * A(W2, i, j, k) * A(RO, i, j, k) * tmp;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k) - OCV * A(W2, i, j, k) * A(TT, i, j, k);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W1, i, j, k) = (A(UU, i + 1, j, k) - A(UU, i - 1, j, k)) * HX * MX(DXXDX, i);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++)
+                A(W3, i, j, k) = (A(WW, i, j, k + 1) - A(WW, i, j, k - 1)) * HZ * MX(DZZDZ, k);
+    tmp = C43 * OCV * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k)
+                    + (A(W3, i, j, k) * A(W3, i, j, k) - A(W1, i, j, k) * A(W3, i, j, k)
+                       - A(W1, i, j, k) * A(W2, i, j, k) - A(W2, i, j, k) * A(W3, i, j, k)) * A(RO, i, j, k) * tmp;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k) - OCV * A(W3, i, j, k) * A(TT, i, j, k);
+    tmp = C13 * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FU, i, j, k) = A(FU, i, j, k) + tmp * (A(W3, i + 1, j, k) - A(W3, i - 1, j, k)) * HX * MX(DXXDX, i);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FV, i, j, k) = A(FV, i, j, k) + tmp * (A(W3, i, j + 1, k) - A(W3, i, j - 1, k)) * HY * MX(DYYDY, j);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 1; i <= nx; i++)
+                A(W1, i, j, k) = (A(UU, i, j, k + 1) - A(UU, i, j, k - 1)) * HZ * MX(DZZDZ, k);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 1; j <= ny; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W2, i, j, k) = (A(VV, i, j, k + 1) - A(VV, i, j, k - 1)) * HZ * MX(DZZDZ, k);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W3, i, j, k) = (A(WW, i, j + 1, k) - A(WW, i, j - 1, k)) * HY * MX(DYYDY, j);
+    tmp = OCV * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k)
+                    + (A(W1, i, j, k) * A(W1, i, j, k) + A(W2, i, j, k) * A(W2, i, j, k)
+                       + A(W3, i, j, k) * A(W3, i, j, k) + 2.0 * A(W2, i, j, k) * A(W3, i, j, k)) * A(RO, i, j, k) * tmp;
+    tmp = C13 * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FW, i, j, k) = A(FW, i, j, k)
+                    + tmp * ((A(W1, i + 1, j, k) - A(W1, i - 1, j, k)) * HX * MX(DXXDX, i)
+                             + (A(W2, i, j + 1, k) - A(W2, i, j - 1, k)) * HY * MX(DYYDY, j));
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(W3, i, j, k) = (A(WW, i + 1, j, k) - A(WW, i - 1, j, k)) * HX * MX(DXXDX, i);
+    tmp = OCV * ORE;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k) + 2.0 * A(W1, i, j, k) * A(W3, i, j, k) * A(RO, i, j, k) * tmp;
+
+    /* Coriolis, 2 Omega = (OMX, 0, OMZ) (sub:2106-2125) */
+    if (s->lrot) {
+        for (k = 2; k <= nz - 1; k++)
+            for (j = 2; j <= ny - 1; j++)
+                for (i = 2; i <= nx - 1; i++) {
+                    A(FU, i, j, k) = A(FU, i, j, k) + OMZ * A(RV, i, j, k);
+                    A(FV, i, j, k) = A(FV, i, j, k) - OMZ * A(RU, i, j, k) + OMX * A(RW, i, j, k);
+                    A(FW, i, j, k) = A(FW, i, j, k) - OMX * A(RV, i, j, k);
+                }
+    }
+
+    if (!s->lmag) return;
+
+    /* MHD block (sub:2129-2634).  The reference uses RU,RV,RW,TT as scratch
+     * for derivatives; they are overwritten here in the same way so that the
+     * observable arrays match the reference after FLUXES. */
+    {
+        double *SU = RU, *SV = RV, *SW = RW, *ST = TT;
+#define LOOP3 for (k = 2; k <= nz - 1; k++) for (j = 2; j <= ny - 1; j++) for (i = 2; i <= nx - 1; i++)
+#define DX_(f) ((A(f, i + 1, j, k) - A(f, i - 1, j, k)) * HX * MX(DXXDX, i))
+#define DY_(f) ((A(f, i, j + 1, k) - A(f, i, j - 1, k)) * HY * MX(DYYDY, j))
+#define DZ_(f) ((A(f, i, j, k + 1) - A(f, i, j, k - 1)) * HZ * MX(DZZDZ, k))
+#define D2X_(f) ((A(f, i + 1, j, k) - A(f, i - 1, j, k)) * HX * MX(D2XXDX2, i) \
+                 + (A(f, i + 1, j, k) - 2.0 * A(f, i, j, k) + A(f, i - 1, j, k)) * H2X * MX(DXXDX, i) * MX(DXXDX, i))
+#define D2Y_(f) ((A(f, i, j + 1, k) - A(f, i, j - 1, k)) * HY * MX(D2YYDY2, j) \
+                 + (A(f, i, j + 1, k) - 2.0 * A(f, i, j, k) + A(f, i, j - 1, k)) * H2Y * MX(DYYDY, j) * MX(DYYDY, j))
+#define D2Z_(f) ((A(f, i, j, k + 1) - A(f, i, j, k - 1)) * HZ * MX(D2ZZDZ2, k) \
+                 + (A(f, i, j, k + 1) - 2.0 * A(f, i, j, k) + A(f, i, j, k - 1)) * H2Z * MX(DZZDZ, k) * MX(DZZDZ, k))
+        /* u-gradients in x; induction terms (sub:2131-2160) */
+        LOOP3 A(SU, i, j, k) = DX_(UU);
+        LOOP3 A(SV, i, j, k) = DX_(VV);
+        LOOP3 A(SW, i, j, k) = DX_(WW);
+        LOOP3 A(W2, i, j, k) = A(BX, i, j, k) * A(SV, i, j, k) - A(BY, i, j, k) * A(SU, i, j, k);
+        LOOP3 A(W3, i, j, k) = A(BX, i, j, k) * A(SW, i, j, k) - A(BZ, i, j, k) * A(SU, i, j, k);
+        /* u-gradients in y (sub:2162-2192) */
+        LOOP3 A(SU, i, j, k) = DY_(UU);
+        LOOP3 A(SV, i, j, k) = DY_(VV);
+        LOOP3 A(SW, i, j, k) = DY_(WW);
+        LOOP3 A(W1, i, j, k) = A(BY, i, j, k) * A(SU, i, j, k) - A(BX, i, j, k) * A(SV, i, j, k);
+        LOOP3 A(W3, i, j, k) = A(W3, i, j, k) + A(BY, i, j, k) * A(SW, i, j, k) - A(BZ, i, j, k) * A(SV, i, j, k);
+        /* u-gradients in z, dw/dz one-sided on the boundary planes (sub:2194-2241) */
+        LOOP3 A(SU, i, j, k) = DZ_(UU);
+        LOOP3 A(SV, i, j, k) = DZ_(VV);
+        LOOP3 A(SW, i, j, k) = DZ_(WW);
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                A(SW, i, j, 2) = (4.0 * A(WW, i, j, 3) - A(WW, i, j, 4)) * HZ * MX(DZZDZ, 2);
+                A(SW, i, j, nz - 1) = (A(WW, i, j, nz - 3) - 4.0 * A(WW, i, j, nz - 2)) * HZ * MX(DZZDZ, nz - 1);
+            }
+        LOOP3 A(W1, i, j, k) = A(W1, i, j, k) + A(BZ, i, j, k) * A(SU, i, j, k) - A(BX, i, j, k) * A(SW, i, j, k);
+        LOOP3 A(W2, i, j, k) = A(W2, i, j, k) + A(BZ, i, j, k) * A(SV, i, j, k) - A(BY, i, j, k) * A(SW, i, j, k);
+        /* B-gradients in x, Lorentz force, Ohmic heating (sub:2243-2319) */
+        LOOP3 A(SU, i, j, k) = DX_(BX);
+        LOOP3 A(SV, i, j, k) = DX_(BY);
+        LOOP3 A(SW, i, j, k) = DX_(BZ);
+        LOOP3 A(ST, i, j, k) = DY_(BX);
+        LOOP3 A(FU, i, j, k) = A(FU, i, j, k) + OBETA * A(BY, i, j, k) * (A(ST, i, j, k) - A(SV, i, j, k))
+                               - OBETA * A(BZ, i, j, k) * A(SW, i, j, k);
+        LOOP3 A(FV, i, j, k) = A(FV, i, j, k) + OBETA * A(BX, i, j, k) * (A(SV, i, j, k) - A(ST, i, j, k));
+        LOOP3 A(FW, i, j, k) = A(FW, i, j, k) + OBETA * A(BX, i, j, k) * A(SW, i, j, k);
+        tmp = OBETA * ORM * OCV;
+        LOOP3 A(FT, i, j, k) = A(FT, i, j, k)
+            + (A(ST, i, j, k) * A(ST, i, j, k) + A(SV, i, j, k) * A(SV, i, j, k) + A(SW, i, j, k) * A(SW, i, j, k)
+               - 2.0 * A(SV, i, j, k) * A(ST, i, j, k)) * A(RO, i, j, k) * tmp;
+        LOOP3 A(W1, i, j, k) = A(W1, i, j, k) - A(UU, i, j, k) * A(SU, i, j, k) - A(VV, i, j, k) * A(ST, i, j, k);
+        LOOP3 A(W2, i, j, k) = A(W2, i, j, k) - A(UU, i, j, k) * A(SV, i, j, k);
+        LOOP3 A(W3, i, j, k) = A(W3, i, j, k) - A(UU, i, j, k) * A(SW, i, j, k);
+        /* resistive diffusion in x and y (sub:2323-2392) */
+        LOOP3 A(SU, i, j, k) = D2X_(BX);
+        LOOP3 A(SV, i, j, k) = D2X_(BY);
+        LOOP3 A(ST, i, j, k) = D2Y_(BX);
+        LOOP3 A(W1, i, j, k) = A(W1, i, j, k) + ORM * (A(SU, i, j, k) + A(ST, i, j, k));
+        LOOP3 A(W2, i, j, k) = A(W2, i, j, k) + ORM * A(SV, i, j, k);
+        LOOP3 A(ST, i, j, k) = D2X_(BZ);
+        LOOP3 A(W3, i, j, k) = A(W3, i, j, k) + ORM * A(ST, i, j, k);
+        /* dBX/dz terms (sub:2394-2433) */
+        LOOP3 A(SU, i, j, k) = DZ_(BX);
+        LOOP3 A(SV, i, j, k) = DY_(BY);
+        LOOP3 A(FU, i, j, k) = A(FU, i, j, k) + OBETA * A(BZ, i, j, k) * A(SU, i, j, k);
+        LOOP3 A(FW, i, j, k) = A(FW, i, j, k) - OBETA * A(BX, i, j, k) * A(SU, i, j, k);
+        tmp = OBETA * ORM * OCV;
+        LOOP3 A(FT, i, j, k) = A(FT, i, j, k)
+            + (A(SU, i, j, k) * A(SU, i, j, k) - 2.0 * A(SU, i, j, k) * A(SW, i, j, k)) * A(RO, i, j, k) * tmp;
+        LOOP3 A(W1, i, j, k) = A(W1, i, j, k) - A(WW, i, j, k) * A(SU, i, j, k);
+        LOOP3 A(W2, i, j, k) = A(W2, i, j, k) - A(VV, i, j, k) * A(SV, i, j, k);
+        /* d2BX/dz2 with one-sided second-order forms on the boundary planes (sub:2443-2471) */
+        LOOP3 A(SU, i, j, k) = D2Z_(BX);
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                A(SU, i, j, 2) = (-(3.0 * A(BX, i, j, 2)) + 4.0 * A(BX, i, j, 3) - A(BX, i, j, 4)) * HZ * MX(D2ZZDZ2, 2)
+                    + (2.0 * A(BX, i, j, 2) - 5.0 * A(BX, i, j, 3) + 4.0 * A(BX, i, j, 4) - A(BX, i, j, 5))
+                      * H2Z * MX(DZZDZ, 2) * MX(DZZDZ, 2);
+                A(SU, i, j, nz - 1) = (3.0 * A(BX, i, j, nz - 1) - 4.0 * A(BX, i, j, nz - 2) + A(BX, i, j, nz - 3))
+                      * HZ * MX(D2ZZDZ2, nz - 1)
+                    + (2.0 * A(BX, i, j, nz - 1) - 5.0 * A(BX, i, j, nz - 2) + 4.0 * A(BX, i, j, nz - 3)
+                       - A(BX, i, j, nz - 4)) * H2Z * MX(DZZDZ, nz - 1) * MX(DZZDZ, nz - 1);
+            }
+        LOOP3 A(SV, i, j, k) = D2Y_(BY);
+        LOOP3 A(W1, i, j, k) = A(W1, i, j, k) + ORM * A(SU, i, j, k);
+        LOOP3 A(W2, i, j, k) = A(W2, i, j, k) + ORM * A(SV, i, j, k);
+        /* dBY/dz and dBZ/dy terms (sub:2496-2548) */
+        LOOP3 A(SV, i, j, k) = DZ_(BY);
+        LOOP3 A(SW, i, j, k) = DY_(BZ);
+        LOOP3 A(SU, i, j, k) = A(SV, i, j, k) - A(SW, i, j, k);
+        LOOP3 A(FV, i, j, k) = A(FV, i, j, k) + OBETA * A(BZ, i, j, k) * A(SU, i, j, k);
+        LOOP3 A(FW, i, j, k) = A(FW, i, j, k) - OBETA * A(BY, i, j, k) * A(SU, i, j, k);
+        tmp = OBETA * ORM * OCV;
+        LOOP3 A(FT, i, j, k) = A(FT, i, j, k)
+            + (A(SV, i, j, k) * A(SV, i, j, k) + A(SW, i, j, k) * A(SW, i, j, k)
+               - 2.0 * A(SV, i, j, k) * A(SW, i, j, k)) * A(RO, i, j, k) * tmp;
+        LOOP3 A(W2, i, j, k) = A(W2, i, j, k) - A(WW, i, j, k) * A(SV, i, j, k);
+        LOOP3 A(W3, i, j, k) = A(W3, i, j, k) - A(VV, i, j, k) * A(SW, i, j, k);
+        /* d2BY/dz2 one-sided on the boundary planes (sub:2552-2580) */
+        LOOP3 A(SV, i, j, k) = D2Z_(BY);
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                A(SV, i, j, 2) = (-(3.0 * A(BY, i, j, 2)) + 4.0 * A(BY, i, j, 3) - A(BY, i, j, 4)) * HZ * MX(D2ZZDZ2, 2)
+                    + (2.0 * A(BY, i, j, 2) - 5.0 * A(BY, i, j, 3) + 4.0 * A(BY, i, j, 4) - A(BY, i, j, 5))
+                      * H2Z * MX(DZZDZ, 2) * MX(DZZDZ, 2);
+                A(SV, i, j, nz - 1) = (3.0 * A(BY, i, j, nz - 1) - 4.0 * A(BY, i, j, nz - 2) + A(BY, i, j, nz - 3))
+                      * HZ * MX(D2ZZDZ2, nz - 1)
+                    + (2.0 * A(BY, i, j, nz - 1) - 5.0 * A(BY, i, j, nz - 2) + 4.0 * A(BY, i, j, nz - 3)
+                       - A(BY, i, j, nz - 4)) * H2Z * MX(DZZDZ, nz - 1) * MX(DZZDZ, nz - 1);
+            }
+        LOOP3 A(SW, i, j, k) = D2Y_(BZ);
+        LOOP3 A(W2, i, j, k) = A(W2, i, j, k) + ORM * A(SV, i, j, k);
+        LOOP3 A(W3, i, j, k) = A(W3, i, j, k) + ORM * A(SW, i, j, k);
+        /* dBZ/dz and d2BZ/dz2 (sub:2605-2633) */
+        LOOP3 A(SW, i, j, k) = DZ_(BZ);
+        LOOP3 A(W3, i, j, k) = A(W3, i, j, k) - A(WW, i, j, k) * A(SW, i, j, k);
+        LOOP3 A(SW, i, j, k) = D2Z_(BZ);
+        LOOP3 A(W3, i, j, k) = A(W3, i, j, k) + ORM * A(SW, i, j, k);
+#undef LOOP3
+#undef DX_
+#undef DY_
+#undef DZ_
+#undef D2X_
+#undef D2Y_
+#undef D2Z_
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * BCON main branches (3dmhdsub.f:2751-2958): stress-free, impenetrable
+ * top/bottom; T fixed (ITC/IZC=0) or fixed gradient (=1); BZ=0 (IBC=0).
+ * ------------------------------------------------------------------------- */
+void orc_bcon(orc_state *s)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    double *RU = s->ru, *RV = s->rv, *RW = s->rw, *RO = s->ro, *TT = s->tt, *BZ = s->bz;
+    const double C13 = s->c13, C23 = s->c23, C43 = s->c43;
+    int i, j;
+    double dz;
+    const int kt = 2, kb = nz - 1;
+
+    /* top (sub:2771-2868) */
+    for (j = 2; j <= ny - 1; j++)
+        for (i = 2; i <= nx - 1; i++)
+            A(RU, i, j, kt) = A(RO, i, j, kt)
+                * (C43 * A(RU, i, j, kt + 1) / A(RO, i, j, kt + 1) - C13 * A(RU, i, j, kt + 2) / A(RO, i, j, kt + 2));
+    for (j = 2; j <= ny - 1; j++)
+        for (i = 2; i <= nx - 1; i++)
+            A(RV, i, j, kt) = A(RO, i, j, kt)
+                * (C43 * A(RV, i, j, kt + 1) / A(RO, i, j, kt + 1) - C13 * A(RV, i, j, kt + 2) / A(RO, i, j, kt + 2));
+    for (j = 2; j <= ny - 1; j++)
+        for (i = 2; i <= nx - 1; i++)
+            A(RW, i, j, kt) = 0.0;
+    if (s->itc == 0) {
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(TT, i, j, kt) = 1.0;
+    } else {
+        dz = s->dzf / MX(s->dzzdz, kt);
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(TT, i, j, kt) = C43 * A(TT, i, j, kt + 1) - C13 * A(TT, i, j, kt + 2) - C23 * dz * s->theta;
+    }
+    if (s->lmag && s->ibc == 0)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(BZ, i, j, kt) = 0.0;
+
+    /* bottom (sub:2873-2955) */
+    for (j = 2; j <= ny - 1; j++)
+        for (i = 2; i <= nx - 1; i++)
+            A(RU, i, j, kb) = A(RO, i, j, kb)
+                * (C43 * A(RU, i, j, kb - 1) / A(RO, i, j, kb - 1) - C13 * A(RU, i, j, kb - 2) / A(RO, i, j, kb - 2));
+    for (j = 2; j <= ny - 1; j++)
+        for (i = 2; i <= nx - 1; i++)
+            A(RV, i, j, kb) = A(RO, i, j, kb)
+                * (C43 * A(RV, i, j, kb - 1) / A(RO, i, j, kb - 1) - C13 * A(RV, i, j, kb - 2) / A(RO, i, j, kb - 2));
+    for (j = 2; j <= ny - 1; j++)
+        for (i = 2; i <= nx - 1; i++)
+            A(RW, i, j, kb) = 0.0;
+    if (s->izc == 0) {
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(TT, i, j, kb) = s->tb;
+    } else {
+        dz = s->dzf / MX(s->dzzdz, kb);
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(TT, i, j, kb) = C43 * A(TT, i, j, kb - 1) - C13 * A(TT, i, j, kb - 2)
+                                  + C23 * dz * s->theta / MX(s->rkapa, kb);
+    }
+    if (s->lmag && s->ibc == 0)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(BZ, i, j, kb) = 0.0;
+}
+
+/* Periodic ghost fill of one field, COMM_MPI semantics with NPEY=1 and no z
+ * neighbours (sub:3131-3138): y first over all x and z, then x over all y, z. */
+static void comm_one(const orc_state *s, double *v)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    int i, j, k;
+    for (k = 1; k <= nz; k++)
+        for (i = 1; i <= nx; i++) {
+            A(v, i, 1, k) = A(v, i, ny - 1, k);
+            A(v, i, ny, k) = A(v, i, 2, k);
+        }
+    for (k = 1; k <= nz; k++)
+        for (j = 1; j <= ny; j++) {
+            A(v, 1, j, k) = A(v, nx - 1, j, k);
+            A(v, nx, j, k) = A(v, 2, j, k);
+        }
+}
+
+/* COMMUNICATE (sub:2980-3029): RU,RV,RW,TT,RO[,BX,BY,BZ]. */
+void orc_communicate(orc_state *s)
+{
+    comm_one(s, s->ru);
+    comm_one(s, s->rv);
+    comm_one(s, s->rw);
+    comm_one(s, s->tt);
+    comm_one(s, s->ro);
+    if (s->lmag) {
+        comm_one(s, s->bx);
+        comm_one(s, s->by);
+        comm_one(s, s->bz);
+    }
+}
+
+/* CFL time step and maximum Mach number (sub:741-873, pointwise branch ISW=1). */
+void orc_cfl(orc_state *s)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    double *RU = s->ru, *RV = s->rv, *RW = s->rw, *RO = s->ro, *TT = s->tt;
+    double *UU = s->uu, *VV = s->vv, *W1 = s->ww1, *W2 = s->ww2, *W3 = s->ww3;
+    double *BX = s->bx, *BY = s->by, *BZ = s->bz;
+    const double GAMMA = s->gamma, OBETA = s->obeta;
+    const double ogamma = 1.0 / GAMMA;
+    double umach = 0.0, wmin[4];
+    int i, j, k, n;
+
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                const double r = 1.0 / A(RO, i, j, k);
+                A(UU, i, j, k) = (A(RU, i, j, k) * A(RU, i, j, k) + A(RV, i, j, k) * A(RV, i, j, k)
+                                  + A(RW, i, j, k) * A(RW, i, j, k)) * (r * r);
+            }
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                const double m = ogamma * A(UU, i, j, k) / A(TT, i, j, k);
+                /* Fortran MAX(a,b) with a non-NaN first argument */
+                if (m > umach) umach = m;
+            }
+    umach = sqrt(umach);
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                const double uu = A(UU, i, j, k);
+                if (s->lmag) {
+                    const double b2 = A(BX, i, j, k) * A(BX, i, j, k) + A(BY, i, j, k) * A(BY, i, j, k)
+                                      + A(BZ, i, j, k) * A(BZ, i, j, k);
+                    A(UU, i, j, k) = uu + GAMMA * A(TT, i, j, k) + b2 * OBETA / A(RO, i, j, k)
+                        + 2.0 * sqrt(uu * (GAMMA * A(TT, i, j, k) + b2 * OBETA / A(RO, i, j, k)));
+                } else {
+                    A(UU, i, j, k) = uu + GAMMA * A(TT, i, j, k) + 2.0 * sqrt(uu * GAMMA * A(TT, i, j, k));
+                }
+            }
+    for (n = 0; n < 4; n++) wmin[n] = INFINITY;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                double dd = MX(s->ddx, i);
+                if (MX(s->ddy, j) < dd) dd = MX(s->ddy, j);
+                if (MX(s->ddz, k) < dd) dd = MX(s->ddz, k);
+                A(W1, i, j, k) = dd / sqrt(A(UU, i, j, k));
+                A(W2, i, j, k) = 0.5 * dd * dd * s->repr * s->cv * A(RO, i, j, k) / MX(s->rkapa, k);
+                A(W3, i, j, k) = 0.375 * dd * dd * s->re * A(RO, i, j, k);
+                if (s->lmag) A(VV, i, j, k) = 0.5 * dd * dd * s->rm;
+                if (A(W1, i, j, k) < wmin[0]) wmin[0] = A(W1, i, j, k);
+                if (A(W2, i, j, k) < wmin[1]) wmin[1] = A(W2, i, j, k);
+                if (A(W3, i, j, k) < wmin[2]) wmin[2] = A(W3, i, j, k);
+                if (s->lmag && A(VV, i, j, k) < wmin[3]) wmin[3] = A(VV, i, j, k);
+            }
+    {
+        double m = wmin[0];
+        if (wmin[1] < m) m = wmin[1];
+        if (wmin[2] < m) m = wmin[2];
+        if (s->lmag && wmin[3] < m) m = wmin[3];
+        s->dt = s->sf * m;
+    }
+    s->umach = umach;
+    for (n = 0; n < 4; n++) s->cflmin[n] = wmin[n];
+}
+
+static void rk_copy(const orc_state *s, double *dst, const double *src)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    int i, j, k;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(dst, i, j, k) = A(src, i, j, k);
+}
+
+/* dst = a + coef * f over the interior */
+static void rk_axpy(const orc_state *s, double *dst, const double *a, double coef, const double *f)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    int i, j, k;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(dst, i, j, k) = A(a, i, j, k) + coef * A(f, i, j, k);
+}
+
+/* q = Z + coef*F for all prognostic fields */
+static void rk_update(orc_state *s, double coef)
+{
+    rk_axpy(s, s->ru, s->zru, coef, s->fu);
+    rk_axpy(s, s->rv, s->zrv, coef, s->fv);
+    rk_axpy(s, s->rw, s->zrw, coef, s->fw);
+    rk_axpy(s, s->ro, s->zro, coef, s->fr);
+    rk_axpy(s, s->tt, s->ztt, coef, s->ft);
+    if (s->lmag) {
+        rk_axpy(s, s->bx, s->zbx, coef, s->ww1);
+        rk_axpy(s, s->by, s->zby, coef, s->ww2);
+        rk_axpy(s, s->bz, s->zbz, coef, s->ww3);
+    }
+}
+
+/* Z = q + coef*F_prev for all prognostic fields */
+static void rk_zreg(orc_state *s, double coef)
+{
+    rk_axpy(s, s->zru, s->ru, coef, s->fu);
+    rk_axpy(s, s->zrv, s->rv, coef, s->fv);
+    rk_axpy(s, s->zrw, s->rw, coef, s->fw);
+    rk_axpy(s, s->zro, s->ro, coef, s->fr);
+    rk_axpy(s, s->ztt, s->tt, coef, s->ft);
+    if (s->lmag) {
+        rk_axpy(s, s->zbx, s->bx, coef, s->ww1);
+        rk_axpy(s, s->zby, s->by, coef, s->ww2);
+        rk_axpy(s, s->zbz, s->bz, coef, s->ww3);
+    }
+}
+
+/* STEP (3dmhdsub.f:687-1176): CFL, then Wray's low-storage RK3. */
+void orc_step(orc_state *s)
+{
+    orc_cfl(s);
+    /* stage 1 (sub:881-973) */
+    rk_copy(s, s->zru, s->ru);
+    rk_copy(s, s->zrv, s->rv);
+    rk_copy(s, s->zrw, s->rw);
+    rk_copy(s, s->zro, s->ro);
+    rk_copy(s, s->ztt, s->tt);
+    if (s->lmag) {
+        rk_copy(s, s->zbx, s->bx);
+        rk_copy(s, s->zby, s->by);
+        rk_copy(s, s->zbz, s->bz);
+    }
+    orc_fluxes(s);
+    rk_update(s, s->gam1 * s->dt);
+    orc_bcon(s);
+    orc_communicate(s);
+    /* stage 2 (sub:977-1071) */
+    rk_zreg(s, s->zeta1 * s->dt);
+    orc_fluxes(s);
+    rk_update(s, s->gam2 * s->dt);
+    orc_bcon(s);
+    orc_communicate(s);
+    /* stage 3 (sub:1075-1169) */
+    rk_zreg(s, s->zeta2 * s->dt);
+    orc_fluxes(s);
+    rk_update(s, s->gam3 * s->dt);
+    orc_bcon(s);
+    orc_communicate(s);
+    /* bookkeeping (sub:1171-1173) */
+    s->nit += 1;
+    s->timc = s->timc + s->dt;
+    s->timt = s->timi + s->timc;
+}

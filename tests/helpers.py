@@ -1,0 +1,109 @@
+"""Shared test helpers: golden-fixture loading, the harness perturbation and
+access to the oracle and the product library."""
+
+from __future__ import annotations
+
+import hashlib
+import importlib
+import os
+import sys
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(REPO, "tests", "golden")
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+FIELDS = ["ru", "rv", "rw", "ro", "tt", "bx", "by", "bz"]
+SMALL = ["hydro16", "mhdrot16", "mhdflux24"]
+ALL = SMALL + ["mhdrot64"]
+
+
+def oracle_mod():
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    try:
+        import oracle  # noqa: E402
+    finally:
+        sys.path.pop(0)
+    oracle.build()
+    return oracle
+
+
+def pkg():
+    return importlib.import_module("rast-3dmhd_amd")
+
+
+def ics():
+    return importlib.import_module("rast-3dmhd_amd.ics")
+
+
+def load(name: str) -> dict:
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    return {k: g[k] for k in g.files}
+
+
+def meta_of(g: dict) -> dict:
+    return {k: v for k, v in g.items() if np.ndim(v) == 0 or k in (
+        "dxxdx", "d2xxdx2", "ddx", "dyydy", "d2yydy2", "ddy", "dzzdz", "d2zzdz2", "ddz",
+        "rkapa", "dkapa", "exx", "wyy", "zee")}
+
+
+def state_sha(fields: dict) -> str:
+    h = hashlib.sha256()
+    for k in FIELDS:
+        h.update(np.asarray(fields[k], dtype=np.float64).tobytes(order="F"))
+    return h.hexdigest()
+
+
+def harness_hash(ig, jg, kg, ic, seed):
+    """HASHR of oracle/refharness.f on int64 arrays of global 1-based indices."""
+    p = np.int64(2147483647)
+    h = (ig * 7919 + jg * 104729 + kg * 1299709 + ic * 15485863 + seed + 12345) % p
+    for _ in range(3):
+        h = (h * 48271) % p
+    return h.astype(np.float64) / float(p) - 0.5
+
+
+def harness_initial_state(g: dict, ampu: float, ampb: float, seed: int) -> dict:
+    """Rebuild the harness's initial state from the committed 1-D profiles:
+    STATIC slicing + B layer + RAN2 noise, the harness perturbation and the
+    first COMMUNICATE."""
+    m = ics()
+    npx, npy, npz = int(g["npx"]), int(g["npy"]), int(g["npz"])
+    lmag = bool(int(g["lmag"]))
+    st = m.static_state(npx, npy, npz, g["prof_t"], g["prof_r"], g["prof_bx"] if lmag else None,
+                        ampt=float(g["ampt"]), obeta=float(g["obeta"]), lmag=lmag)
+    ig, jg, kg = np.meshgrid(np.arange(2, npx + 2, dtype=np.int64),
+                             np.arange(2, npy + 2, dtype=np.int64),
+                             np.arange(2, npz + 2, dtype=np.int64), indexing="ij")
+    sl = (slice(1, -1),) * 3
+    ro = st["ro"][sl]
+    st["ru"][sl] = st["ru"][sl] + ro * ampu * harness_hash(ig, jg, kg, 1, seed)
+    st["rv"][sl] = st["rv"][sl] + ro * ampu * harness_hash(ig, jg, kg, 2, seed)
+    inner = (kg != 2) & (kg != npz + 1)
+    st["rw"][sl] = np.where(inner, st["rw"][sl] + ro * ampu * harness_hash(ig, jg, kg, 3, seed), st["rw"][sl])
+    if lmag:
+        st["bx"][sl] = st["bx"][sl] + ampb * harness_hash(ig, jg, kg, 4, seed)
+        st["by"][sl] = st["by"][sl] + ampb * harness_hash(ig, jg, kg, 5, seed)
+        st["bz"][sl] = np.where(inner, st["bz"][sl] + ampb * harness_hash(ig, jg, kg, 6, seed), st["bz"][sl])
+    for k in FIELDS:
+        m.periodic_fill(st[k])
+    return st
+
+
+def harness_params(name: str):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    try:
+        import refconfigs
+    finally:
+        sys.path.pop(0)
+    c = refconfigs.CONFIGS[name]
+    return float(c["ampu"].replace("D", "e")), float(c["ampb"].replace("D", "e")), int(c["seed"])
+
+
+def initial_state(name: str) -> dict:
+    g = load(name)
+    if int(g["full"]):
+        return {k: g[f"init_{k}"].copy(order="F") for k in FIELDS}
+    return harness_initial_state(g, *harness_params(name))
