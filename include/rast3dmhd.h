@@ -1,0 +1,127 @@
+/*
+ * rast3dmhd.h -- C-ABI of the MI355X explicit compressible-MHD time step.
+ *
+ * This is the drop-in boundary for the reference's SUBROUTINE STEP
+ * (3dmhdsub.f:687-1176), which takes no arguments and exchanges everything
+ * through COMMON blocks (3dmhd.f:51-71).  A Fortran STEP shim with the same
+ * name and COMMONs (rast-3dmhd_amd/fortran/step_shim.f) binds these entry
+ * points with ISO_C_BINDING; the driver is unchanged.
+ *
+ * Conventions
+ *  - Every array argument is a host pointer to FP64 data in the reference's
+ *    padded Fortran order: 3-D fields are (NX,NY,NZ) = (NPX+2,NPY+2,NPZ+2),
+ *    x fastest, one ghost layer per side; metric arrays have length NX, NY
+ *    or NZ.  The library never keeps a host pointer past a call.
+ *  - Every entry point returns 0 on success and a nonzero MHD_ERR_* code on
+ *    failure; mhd_last_error() then describes it.  The reference prints and
+ *    STOPs instead (e.g. 3dmhdsub.f:1262-1265, 2942-2946); the shim does that.
+ *  - One HIP stream per handle, one host thread per handle.
+ *  - The whole domain is resident on the current HIP device.  The
+ *    reference's MPI decomposition (3dmhd.f:87-88, 3dmhdsub.f:3031-3141)
+ *    collapses to periodic index wrap in x/y and untouched z ghost planes.
+ */
+#ifndef RAST3DMHD_H
+#define RAST3DMHD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MHD_OK = 0,
+    MHD_ERR_ARG = 1,        /* bad argument or size */
+    MHD_ERR_SCOPE = 2,      /* configuration outside the supported branches */
+    MHD_ERR_HIP = 3,        /* HIP runtime error */
+    MHD_ERR_STATE = 4,      /* call out of order (e.g. step before upload) */
+    MHD_ERR_NONFINITE = 5   /* DT or UMACH not finite after a step */
+};
+
+/* Everything STEP reads from COMMON besides the arrays.  Replaces the
+ * PARAMETER block (3dmhdparam.f: NPX,NPY,NPZ, LMAG, LROT, IZCON, ITCON,
+ * IBCON) and the scalars of /GRID/, /RUNGKU/, /CPAR/, /CROT/, /CMAG/,
+ * /CPEN/ and /CPER/ (3dmhd.f:62-71).  The float32-folded constants (C13,
+ * GAM1, GAMMA, ...) must be passed as the driver computed them, never
+ * retyped. */
+typedef struct mhd_desc {
+    int npx, npy, npz;          /* interior points; z includes both boundary planes */
+    int lmag, lrot;             /* LMAG, LROT */
+    int izcon, itcon, ibcon;    /* bottom T (0 fixed, 1 flux), top T (0/1), B (0: BZ=0) */
+    int reserved;
+    double hx, h2x, hy, h2y, hz, h2z, c13, c23, c43;   /* /GRID/ */
+    double gam1, gam2, gam3, zeta1, zeta2;              /* /RUNGKU/ */
+    double cv, ocv, ore, re, repr, theta, grav, sf, gamma; /* /CPAR/ */
+    double omx, omz;                                    /* /CROT/ */
+    double orm, rm, obeta;                              /* /CMAG/ */
+    double tb;                                          /* /CPEN/ TB */
+    double tp;                                          /* /CPER/ TP (must be 0) */
+    double dzf;             /* float32 1.0E00/FLOAT(NPZ-1) widened (BCON, sub:2856/2929) */
+} mhd_desc;
+
+typedef struct mhd_ctx mhd_t;
+
+/* Allocates the device state for one box.  Replaces the compile-time sizing
+ * of COMMON /BIG/ (3dmhd.f:22-56). */
+int mhd_create(const mhd_desc *desc, mhd_t **out);
+
+/* Uploads the metric arrays of /AJACOBI/ and RKAPA/DKAPA of /CPEN/
+ * (3dmhd.f:58-59, 65): lengths NX, NY, NZ (padded). */
+int mhd_set_metrics(mhd_t *h,
+                    const double *dxxdx, const double *d2xxdx2, const double *ddx,
+                    const double *dyydy, const double *d2yydy2, const double *ddy,
+                    const double *dzzdz, const double *d2zzdz2, const double *ddz,
+                    const double *rkapa, const double *dkapa);
+
+/* Uploads the prognostic arrays RU,RV,RW,RO,TT[,BX,BY,BZ] of /BIG/.  The
+ * x/y ghosts are ignored (the library wraps periodically, which is what
+ * COMMUNICATE establishes, 3dmhdsub.f:3131-3138); the z ghost planes are
+ * kept.  bx, by, bz may be NULL when LMAG is off. */
+int mhd_upload(mhd_t *h, const double *ru, const double *rv, const double *rw,
+               const double *ro, const double *tt,
+               const double *bx, const double *by, const double *bz);
+
+/* Downloads the prognostic arrays, ghosts filled as COMMUNICATE leaves them
+ * after STEP.  bx, by, bz may be NULL. */
+int mhd_download(mhd_t *h, double *ru, double *rv, double *rw, double *ro, double *tt,
+                 double *bx, double *by, double *bz);
+
+/* One full time step: CFL/Mach reduction, three Wray RK3 stages with BCON
+ * and the periodic ghost semantics (3dmhdsub.f:741-1173).  Returns the DT
+ * used and the maximum Mach number UMACH computed from the state at entry
+ * (COMMON /CTIM/ DT, /TRACE/ UMACH).  Synchronous. */
+int mhd_step(mhd_t *h, double *dt, double *umach);
+
+/* n steps back to back with no host synchronisation in between (the
+ * driver's DO 1000 loop without dumps); dt/umach of the last step. */
+int mhd_steps(mhd_t *h, int n, double *dt, double *umach);
+
+/* Step counter and clocks of /ITER/ NIT and /CTIM/ TIMC, TIMT, TIMI as STEP
+ * updates them (3dmhdsub.f:1171-1173). */
+int mhd_get_clock(mhd_t *h, int *nit, double *timc, double *timt);
+int mhd_set_clock(mhd_t *h, int nit, double timc, double timi);
+
+/* One evaluation of the right-hand side on the uploaded state, without
+ * stepping: FU,FV,FW,FR,FT and (LMAG) WW1,WW2,WW3 = dB/dt, as FLUXES
+ * leaves them (3dmhdsub.f:1178-2637).  Interior values only; ghost entries
+ * of the output arrays are set to zero.  ww1..ww3 may be NULL. */
+int mhd_fluxes(mhd_t *h, double *fu, double *fv, double *fw, double *fr, double *ft,
+               double *ww1, double *ww2, double *ww3);
+
+/* Measurement hook (no counterpart in the reference): runs n steps like
+ * mhd_steps with HIP events around every launch on the handle's stream and
+ * returns the mean duration in milliseconds of one step, one RK stage
+ * kernel launch (averaged over the three stages), one CFL reduction and one
+ * boundary-plane launch.  Any output pointer may be NULL. */
+int mhd_time_steps(mhd_t *h, int n, double *ms_step, double *ms_stage, double *ms_cfl,
+                   double *ms_bcon);
+
+/* Human-readable description of the last error on this handle (or of the
+ * last failed mhd_create when h is NULL). */
+const char *mhd_last_error(const mhd_t *h);
+
+/* Frees the device state. */
+void mhd_destroy(mhd_t *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -60,7 +60,8 @@ def build(name: str, cfg: dict | None = None, main_exe: bool = False) -> str:
     cfg = cfg or refconfigs.CONFIGS[name]
     d = os.path.join(REF_OUT, name)
     os.makedirs(d, exist_ok=True)
-    mkparam.write_config(d, cfg["params"], par=cfg["par"], ipar=cfg.get("ipar"))
+    mkparam.write_config(d, cfg["params"], par=cfg["par"], ipar=cfg.get("ipar"),
+                         env_steps=main_exe)
     links = {"3dmhdsub.f": os.path.join(REF_SRC, "3dmhdsub.f"),
              "refharness.f": os.path.join(HERE, "refharness.f")}
     if main_exe:
@@ -84,6 +85,19 @@ def build(name: str, cfg: dict | None = None, main_exe: bool = False) -> str:
     if main_exe:
         _run([FC, "3dmhd.o", "3dmhdsub.o", "3dmhdset.o", *libs, "-o", "3dmhd.exe"], d)
     return d
+
+
+def build_baseline() -> str:
+    """Compile the reference main program for the throughput workload
+    (refconfigs.BASELINE), used as the CPU baseline of bench.py.  Skips the
+    compile when the executable is newer than this recipe."""
+    name, cfg = refconfigs.BASELINE_NAME, refconfigs.baseline_config()
+    exe = os.path.join(REF_OUT, name, "3dmhd.exe")
+    if os.path.exists(exe) and os.path.getmtime(exe) > max(
+            os.path.getmtime(__file__), os.path.getmtime(os.path.join(HERE, "refconfigs.py"))):
+        return exe
+    build(name, cfg, main_exe=True)
+    return exe
 
 
 def run_harness(name: str, cfg: dict | None = None) -> str:

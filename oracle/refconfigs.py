@@ -12,6 +12,32 @@ NPEY=1), and NPEZ>=2 as the reference requires (3dmhdparam.f:11).
 # literal like every PAR value.
 ANG45 = "0.7853982E00"
 
+BASELINE_NAME = "bench512"
+
+
+def baseline_config(npx=512, npy=512, npz=256, npey=8, npez=2):
+    """The throughput workload as a reference build: 512x512x256, MHD with a
+    45-degree tilted rotation axis, density contrast 30, Ra=1e6, Pr=1, Pm=2,
+    Ta=1e6 (parameter mapping of rast-3dmhd_amd/config.physics), run on
+    npey*npez MPI ranks (16 host cores by default)."""
+    import importlib.util
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location(
+        "_cfg", os.path.join(os.path.dirname(here), "rast-3dmhd_amd", "config.py"))
+    cfg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(cfg)
+    ph = cfg.physics(contrast=30.0, ra=1e6, pr=1.0, pm=2.0, ta=1e6, tilt_deg=45.0)
+    lit = lambda x: f"{x:.8E}"
+    par = {k: lit(v) for k, v in ph.items()}
+    par.update({20: "1.0E-04", 21: "1.0E-03", 22: "0.3E00", 23: "0.5E00"})
+    return dict(
+        params=dict(NPX=npx, NPY=npy, NPZ=npz, NPEY=npey, NPEZ=npez, LMAG=".TRUE.", LROT=".TRUE.",
+                    IZCON=1, ITCON=0),
+        par=par, ipar={3: 3, 4: 100000},
+    )
+
+
 CONFIGS = {
     # Hydrodynamic, no rotation, stretched z, fixed T top / fixed flux bottom.
     "hydro16": dict(

@@ -1,1 +1,181 @@
-"""MI355X-native explicit compressible-MHD time step (Rast 3dmhd STEP)."""
+"""MI355X-native explicit compressible-MHD time step (Rast 3dmhd STEP).
+
+The product is the C-ABI shared library librast3dmhd.so built from csrc/
+(declared in include/rast3dmhd.h) and the Fortran STEP shim in fortran/.
+This module is the Python binding of that C-ABI used by the tests and the
+benchmark: it loads the library, fills the mhd_desc struct from the
+driver's COMMON scalars and moves padded Fortran-order arrays in and out.
+There is no fallback: if the HIP library is missing, loading raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librast3dmhd.so")
+FIELDS = ("ru", "rv", "rw", "ro", "tt", "bx", "by", "bz")
+RHS = ("fu", "fv", "fw", "fr", "ft", "ww1", "ww2", "ww3")
+
+MHD_OK, MHD_ERR_ARG, MHD_ERR_SCOPE, MHD_ERR_HIP, MHD_ERR_STATE, MHD_ERR_NONFINITE = range(6)
+
+_DESC_INTS = ["npx", "npy", "npz", "lmag", "lrot", "izcon", "itcon", "ibcon", "reserved"]
+_DESC_DBLS = ["hx", "h2x", "hy", "h2y", "hz", "h2z", "c13", "c23", "c43",
+              "gam1", "gam2", "gam3", "zeta1", "zeta2",
+              "cv", "ocv", "ore", "re", "repr", "theta", "grav", "sf", "gamma",
+              "omx", "omz", "orm", "rm", "obeta", "tb", "tp", "dzf"]
+METRICS = ("dxxdx", "d2xxdx2", "ddx", "dyydy", "d2yydy2", "ddy",
+           "dzzdz", "d2zzdz2", "ddz", "rkapa", "dkapa")
+
+
+class MhdDesc(C.Structure):
+    _fields_ = [(k, C.c_int) for k in _DESC_INTS] + [(k, C.c_double) for k in _DESC_DBLS]
+
+
+class MhdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+_P = C.POINTER(C.c_double)
+
+
+def load(path: str | None = None):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() "
+                           "(the HIP library is the only implementation)")
+    lib = C.CDLL(path)
+    lib.mhd_create.argtypes = [C.POINTER(MhdDesc), C.POINTER(C.c_void_p)]
+    lib.mhd_set_metrics.argtypes = [C.c_void_p] + [_P] * 11
+    lib.mhd_upload.argtypes = [C.c_void_p] + [_P] * 8
+    lib.mhd_download.argtypes = [C.c_void_p] + [_P] * 8
+    lib.mhd_step.argtypes = [C.c_void_p, _P, _P]
+    lib.mhd_steps.argtypes = [C.c_void_p, C.c_int, _P, _P]
+    lib.mhd_get_clock.argtypes = [C.c_void_p, C.POINTER(C.c_int), _P, _P]
+    lib.mhd_set_clock.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+    lib.mhd_fluxes.argtypes = [C.c_void_p] + [_P] * 8
+    lib.mhd_time_steps.argtypes = [C.c_void_p, C.c_int, _P, _P, _P, _P]
+    lib.mhd_last_error.argtypes = [C.c_void_p]
+    lib.mhd_last_error.restype = C.c_char_p
+    lib.mhd_destroy.argtypes = [C.c_void_p]
+    lib.mhd_destroy.restype = None
+    for fn in ("mhd_create", "mhd_set_metrics", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
+               "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps"):
+        getattr(lib, fn).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def make_desc(meta: dict) -> MhdDesc:
+    """mhd_desc from a dict of COMMON scalars (names as in the reference,
+    lower case; npx/npy/npz are interior sizes)."""
+    d = MhdDesc()
+    d.npx, d.npy, d.npz = int(meta["npx"]), int(meta["npy"]), int(meta["npz"])
+    d.lmag, d.lrot = int(meta.get("lmag", 0)), int(meta.get("lrot", 0))
+    d.izcon = int(meta.get("izc", meta.get("izcon", 0)))
+    d.itcon = int(meta.get("itc", meta.get("itcon", 0)))
+    d.ibcon = int(meta.get("ibc", meta.get("ibcon", 0)))
+    for k in _DESC_DBLS:
+        setattr(d, k, float(meta.get(k, 0.0)))
+    return d
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+class Box:
+    """One resident box on the current GPU, driven through the C-ABI."""
+
+    def __init__(self, meta: dict):
+        self.lib = load()
+        self.meta = meta
+        self.desc = make_desc(meta)
+        self.shape = (self.desc.npx + 2, self.desc.npy + 2, self.desc.npz + 2)
+        self.lmag = bool(self.desc.lmag)
+        h = C.c_void_p()
+        rc = self.lib.mhd_create(C.byref(self.desc), C.byref(h))
+        if rc:
+            raise MhdError(rc, self.lib.mhd_last_error(None).decode())
+        self.h = h
+        self._metrics = [np.ascontiguousarray(meta[k], dtype=np.float64) for k in METRICS]
+        self._check(self.lib.mhd_set_metrics(self.h, *[_ptr(a) for a in self._metrics]))
+
+    def _check(self, rc):
+        if rc:
+            raise MhdError(rc, self.lib.mhd_last_error(self.h).decode())
+
+    def _padded(self, a):
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"array shape {a.shape}, expected {self.shape}")
+        return np.asfortranarray(a)
+
+    def upload(self, fields: dict):
+        arrs = [self._padded(fields[k]) if (k in fields and (self.lmag or k not in ("bx", "by", "bz"))) else None
+                for k in FIELDS]
+        self._check(self.lib.mhd_upload(self.h, *[_ptr(a) for a in arrs]))
+
+    def download(self) -> dict:
+        out = {k: np.zeros(self.shape, order="F") for k in FIELDS}
+        ptrs = [_ptr(out[k]) if (self.lmag or k not in ("bx", "by", "bz")) else None for k in FIELDS]
+        self._check(self.lib.mhd_download(self.h, *ptrs))
+        return out
+
+    def step(self):
+        dt, um = C.c_double(), C.c_double()
+        self._check(self.lib.mhd_step(self.h, C.byref(dt), C.byref(um)))
+        return dt.value, um.value
+
+    def steps(self, n: int):
+        dt, um = C.c_double(), C.c_double()
+        self._check(self.lib.mhd_steps(self.h, int(n), C.byref(dt), C.byref(um)))
+        return dt.value, um.value
+
+    def time_steps(self, n: int) -> dict:
+        """Mean milliseconds per step and per kernel launch over n steps,
+        from HIP events on the library's stream."""
+        v = [C.c_double() for _ in range(4)]
+        self._check(self.lib.mhd_time_steps(self.h, int(n), *[C.byref(x) for x in v]))
+        return dict(step=v[0].value, stage=v[1].value, cfl=v[2].value, bcon=v[3].value)
+
+    def clock(self):
+        nit, timc, timt = C.c_int(), C.c_double(), C.c_double()
+        self._check(self.lib.mhd_get_clock(self.h, C.byref(nit), C.byref(timc), C.byref(timt)))
+        return nit.value, timc.value, timt.value
+
+    def set_clock(self, nit: int, timc: float, timi: float):
+        self._check(self.lib.mhd_set_clock(self.h, int(nit), float(timc), float(timi)))
+
+    def fluxes(self) -> dict:
+        out = {k: np.zeros(self.shape, order="F") for k in RHS}
+        ptrs = [_ptr(out[k]) if (self.lmag or k not in ("ww1", "ww2", "ww3")) else None for k in RHS]
+        self._check(self.lib.mhd_fluxes(self.h, *ptrs))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,73 @@
+// Device-side data structures shared by the kernels and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rmhd {
+
+// Field order in every device register (q, Z, F).
+enum Field { F_RU = 0, F_RV, F_RW, F_RO, F_TT, F_BX, F_BY, F_BZ, NFIELD };
+
+// Scalars of the step, all as the driver computed them, plus products the
+// reference forms once per FLUXES call (e.g. TMP=OCV*ORE), computed on the
+// host with the same association.
+struct Params {
+    int npx, npy, npz;          // interior sizes
+    int lmag, lrot, izc, itc, ibc;
+    long long plane;            // npx*npy
+    long long fsize;            // plane*(npz+2): one field incl. z ghost planes
+    double hx, h2x, hy, h2y, hz, h2z, c13, c23, c43;
+    double gam1, gam2, gam3, zeta1, zeta2;
+    double cv, ocv, ore, re, repr, theta, grav, sf, gamma;
+    double omx, omz, orm, rm, obeta, tb, dzf;
+    double ogamma;              // 1/GAMMA (sub:744)
+    double t_cond;              // OCV/REPR (sub:1584)
+    double t_visc;              // OCV*ORE (sub:1904)
+    double t_visc43;            // C43*OCV*ORE (sub:1962)
+    double t_cross;             // C13*ORE (sub:1930)
+    double t_ohm;               // OBETA*ORM*OCV (sub:2287)
+};
+
+// Device metric table: padded arrays, same indexing as the reference
+// (element I of the reference is at index I-1).
+struct Metrics {
+    const double *dxxdx, *d2xxdx2, *ddx;
+    const double *dyydy, *d2yydy2, *ddy;
+    const double *dzzdz, *d2zzdz2, *ddz;
+    const double *rkapa, *dkapa;
+};
+
+// Per-step device scalars.  The reductions keep order-preserving integer
+// keys of the doubles so that min/max can use 64-bit integer atomics.
+struct DevScalars {
+    double dt, umach;
+    double cflmin[4];
+    double timc, timi, timt;
+    unsigned long long key_mach;      // max of OGAMMA*UU/TT
+    unsigned long long key_min[4];    // mins of the CFL limits
+    int nit;
+    int nonfinite;
+};
+
+__host__ __device__ inline unsigned long long dkey(double v) {
+    unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double dunkey(unsigned long long k) {
+    unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __builtin_bit_cast(double, b);
+}
+
+// Launchers (mhd_kernels.hip).
+hipError_t launch_stage(int mode, const Params &p, const Metrics &m, const double *qin, double *qout,
+                        double *z, double *fout, DevScalars *sc, hipStream_t s);
+hipError_t launch_bcon(const Params &p, const Metrics &m, const double *qin, double *qout,
+                       hipStream_t s);
+hipError_t launch_cfl(const Params &p, const Metrics &m, const double *q, DevScalars *sc,
+                      hipStream_t s);
+hipError_t launch_pack(const Params &p, const double *padded, double *field, hipStream_t s);
+hipError_t launch_unpack(const Params &p, const double *field, double *padded, hipStream_t s);
+hipError_t launch_clock(DevScalars *sc, hipStream_t s);
+
+}  // namespace rmhd

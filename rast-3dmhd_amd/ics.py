@@ -100,6 +100,62 @@ def static_state(npx: int, npy: int, npz: int, col_t, col_r, col_bx=None,
     return st
 
 
+def vector_potential_field(meta: dict, kmax: int, amp: float, seed: int):
+    """Seed field B = curl A from a random vector potential with integer
+    wavenumbers |k| <= kmax (horizontal Fourier modes of the periodic box,
+    vertical modes sin(pi m z) for Ax, Ay and cos(pi m z) for Az, m >= 1).
+    The curl uses the step's own centred, metric-weighted differences, so the
+    centred divergence of B vanishes in the interior to round-off.  BZ is set
+    to zero on the boundary planes (the IBCON=0 condition).  Scaled so that
+    max|B| = amp.  Returns padded (bx, by, bz) with periodic ghosts."""
+    npx, npy, npz = int(meta["npx"]), int(meta["npy"]), int(meta["npz"])
+    rng = np.random.default_rng(seed)
+    zee = np.asarray(meta["zee"], dtype=np.float64)
+    zl = zee[-2] - zee[1]
+    zc = np.empty(npz + 2)
+    zc[1:-1] = (zee[1:-1] - zee[1]) / zl
+    zc[0] = 2 * zc[1] - zc[2]
+    zc[-1] = 2 * zc[-2] - zc[-3]
+    kx = np.fft.fftfreq(npx, 1.0 / npx)[:, None]
+    ky = np.fft.rfftfreq(npy, 1.0 / npy)[None, :]
+    A = [np.zeros((npx, npy, npz + 2)) for _ in range(3)]
+    for m in range(1, kmax + 1):
+        sel = (kx ** 2 + ky ** 2 + m ** 2) <= kmax ** 2
+        if not sel.any():
+            continue
+        for c in range(3):
+            coef = (rng.standard_normal(sel.shape) + 1j * rng.standard_normal(sel.shape)) * sel
+            plane = np.fft.irfft2(coef, s=(npx, npy)) * (npx * npy)
+            prof = np.cos(np.pi * m * zc) if c == 2 else np.sin(np.pi * m * zc)
+            A[c] += plane[:, :, None] * prof[None, None, :]
+    hx, hy, hz = float(meta["hx"]), float(meta["hy"]), float(meta["hz"])
+    dx = np.asarray(meta["dxxdx"])[1:-1][:, None, None]
+    dy = np.asarray(meta["dyydy"])[1:-1][None, :, None]
+    dz = np.asarray(meta["dzzdz"])[1:-1][None, None, :]
+
+    def ddx(f):
+        return (np.roll(f, -1, 0) - np.roll(f, 1, 0))[:, :, 1:-1] * hx * dx
+
+    def ddy(f):
+        return (np.roll(f, -1, 1) - np.roll(f, 1, 1))[:, :, 1:-1] * hy * dy
+
+    def ddz(f):
+        return (f[:, :, 2:] - f[:, :, :-2]) * hz * dz
+
+    bx = ddy(A[2]) - ddz(A[1])
+    by = ddz(A[0]) - ddx(A[2])
+    bz = ddx(A[1]) - ddy(A[0])
+    bz[:, :, 0] = 0.0
+    bz[:, :, -1] = 0.0
+    scale = amp / max(np.abs(bx).max(), np.abs(by).max(), np.abs(bz).max(), 1e-300)
+    out = []
+    for b in (bx, by, bz):
+        p = np.zeros((npx + 2, npy + 2, npz + 2), order="F")
+        p[1:-1, 1:-1, 1:-1] = b * scale
+        out.append(periodic_fill(p))
+    return tuple(out)
+
+
 def polytrope_profiles(zee: np.ndarray, theta: float, grav: float):
     """Closed-form polytrope T = 1 + theta z, rho = T**(grav/theta - 1) on a
     padded z column (ghost planes repeat the boundary values).  STATIC
