@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -17,10 +18,6 @@
 #include "mhd_kernels.h"
 
 using namespace rmhd;
-
-namespace rmhd {
-hipError_t launch_unpack_rhs(const Params &p, const double *field, double *padded, hipStream_t s);
-}
 
 struct mhd_ctx {
     int dev = 0;
@@ -36,6 +33,7 @@ struct mhd_ctx {
     DevScalars *sc = nullptr;
     int cur = 0;
     bool have_metrics = false, have_state = false;
+    bool cfl_valid = false;             // sc->dt/umach describe q[cur]
     char err[512] = {0};
 };
 
@@ -89,6 +87,8 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
     p.npx = d->npx; p.npy = d->npy; p.npz = d->npz;
     p.lmag = d->lmag ? 1 : 0; p.lrot = d->lrot ? 1 : 0;
     p.izc = d->izcon; p.itc = d->itcon; p.ibc = d->ibcon;
+    p.tile_y = 4;
+    if (const char *ty = getenv("RAST3DMHD_TILE_Y")) p.tile_y = atoi(ty) == 8 ? 8 : 4;
     p.plane = (long long)d->npx * d->npy;
     p.fsize = p.plane * (d->npz + 2);
     p.hx = d->hx; p.h2x = d->h2x; p.hy = d->hy; p.h2y = d->h2y; p.hz = d->hz; p.h2z = d->h2z;
@@ -179,6 +179,7 @@ extern "C" int mhd_upload(mhd_t *h, const double *ru, const double *rv, const do
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->cur = 0;
     h->have_state = true;
+    h->cfl_valid = false;
     return MHD_OK;
 }
 
@@ -192,25 +193,32 @@ extern "C" int mhd_download(mhd_t *h, double *ru, double *rv, double *rw, double
     HIPCHK(h, hipSetDevice(h->dev));
     for (int f = 0; f < nf; ++f) {
         if (!dst[f]) continue;
-        HIPCHK(h, launch_unpack(h->p, h->q[h->cur] + (size_t)f * h->p.fsize, h->stage, h->stream));
+        HIPCHK(h, launch_unpack(h->p, h->q[h->cur] + (size_t)f * h->p.fsize, h->stage, 1, h->stream));
         HIPCHK(h, hipMemcpyAsync(dst[f], h->stage, pbytes, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
     }
     return MHD_OK;
 }
 
+// One STEP.  The CFL limits of the state a step starts from are reduced by
+// the previous step's stage-3 and boundary kernels; only the first step
+// after an upload runs the stand-alone reduction.
 static int enqueue_step(mhd_t *h) {
     const Params &p = h->p;
     double *qa = h->q[h->cur], *qb = h->q[1 - h->cur];
-    HIPCHK(h, launch_cfl(p, h->m, qa, h->sc, h->stream));
+    if (!h->cfl_valid) {
+        HIPCHK(h, launch_cfl(p, h->m, qa, h->sc, h->stream));
+        HIPCHK(h, launch_cfl_final(p, h->sc, false, h->stream));
+    }
     HIPCHK(h, launch_stage(1, p, h->m, qa, qb, h->z, nullptr, h->sc, h->stream));
-    HIPCHK(h, launch_bcon(p, h->m, qa, qb, h->stream));
+    HIPCHK(h, launch_bcon(p, h->m, qa, qb, h->sc, false, h->stream));
     HIPCHK(h, launch_stage(2, p, h->m, qb, qa, h->z, nullptr, h->sc, h->stream));
-    HIPCHK(h, launch_bcon(p, h->m, qb, qa, h->stream));
+    HIPCHK(h, launch_bcon(p, h->m, qb, qa, h->sc, false, h->stream));
     HIPCHK(h, launch_stage(3, p, h->m, qa, qb, h->z, nullptr, h->sc, h->stream));
-    HIPCHK(h, launch_bcon(p, h->m, qa, qb, h->stream));
-    HIPCHK(h, launch_clock(h->sc, h->stream));
+    HIPCHK(h, launch_bcon(p, h->m, qa, qb, h->sc, true, h->stream));
+    HIPCHK(h, launch_cfl_final(p, h->sc, true, h->stream));
     h->cur = 1 - h->cur;
+    h->cfl_valid = true;
     return MHD_OK;
 }
 
@@ -218,8 +226,8 @@ static int finish(mhd_t *h, double *dt, double *umach) {
     DevScalars s;
     HIPCHK(h, hipMemcpyAsync(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (dt) *dt = s.dt;
-    if (umach) *umach = s.umach;
+    if (dt) *dt = s.dt_used;
+    if (umach) *umach = s.umach_used;
     if (s.nonfinite)
         return fail(h, MHD_ERR_NONFINITE, "non-finite CFL limit or Mach number (DT=%g, UMACH=%g)", s.dt, s.umach);
     return MHD_OK;
@@ -275,20 +283,24 @@ extern "C" int mhd_time_steps(mhd_t *h, int n, double *ms_step, double *ms_stage
     for (int s = 0; s < n && e == hipSuccess; ++s) {
         double *qa = h->q[h->cur], *qb = h->q[1 - h->cur];
         double *in[3] = {qa, qb, qa}, *out[3] = {qb, qa, qb};
-        hipEventRecord(ev[ie++], h->stream);
-        e = launch_cfl(p, h->m, qa, h->sc, h->stream);
-        hipEventRecord(ev[ie++], h->stream);
-        for (int st = 0; st < 3 && e == hipSuccess; ++st) {
-            hipEventRecord(ev[ie++], h->stream);
-            e = launch_stage(st + 1, p, h->m, in[st], out[st], h->z, nullptr, h->sc, h->stream);
-            hipEventRecord(ev[ie++], h->stream);
-            if (e != hipSuccess) break;
-            hipEventRecord(ev[ie++], h->stream);
-            e = launch_bcon(p, h->m, in[st], out[st], h->stream);
-            hipEventRecord(ev[ie++], h->stream);
+        (void)hipEventRecord(ev[ie++], h->stream);
+        if (!h->cfl_valid) {
+            e = launch_cfl(p, h->m, qa, h->sc, h->stream);
+            if (e == hipSuccess) e = launch_cfl_final(p, h->sc, false, h->stream);
         }
-        if (e == hipSuccess) e = launch_clock(h->sc, h->stream);
+        (void)hipEventRecord(ev[ie++], h->stream);
+        for (int st = 0; st < 3 && e == hipSuccess; ++st) {
+            (void)hipEventRecord(ev[ie++], h->stream);
+            e = launch_stage(st + 1, p, h->m, in[st], out[st], h->z, nullptr, h->sc, h->stream);
+            (void)hipEventRecord(ev[ie++], h->stream);
+            if (e != hipSuccess) break;
+            (void)hipEventRecord(ev[ie++], h->stream);
+            e = launch_bcon(p, h->m, in[st], out[st], h->sc, st == 2, h->stream);
+            (void)hipEventRecord(ev[ie++], h->stream);
+        }
+        if (e == hipSuccess) e = launch_cfl_final(p, h->sc, true, h->stream);
         h->cur = 1 - h->cur;
+        h->cfl_valid = true;
     }
     if (e == hipSuccess) e = hipEventRecord(ev[1], h->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
@@ -354,7 +366,7 @@ extern "C" int mhd_fluxes(mhd_t *h, double *fu, double *fv, double *fw, double *
     const size_t pbytes = (size_t)(h->p.npx + 2) * (h->p.npy + 2) * (h->p.npz + 2) * sizeof(double);
     for (int f = 0; f < nf; ++f) {
         if (!dst[f]) continue;
-        HIPCHK(h, launch_unpack_rhs(h->p, h->f + (size_t)f * h->p.fsize, h->stage, h->stream));
+        HIPCHK(h, launch_unpack(h->p, h->f + (size_t)f * h->p.fsize, h->stage, 0, h->stream));
         HIPCHK(h, hipMemcpyAsync(dst[f], h->stage, pbytes, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
     }

@@ -15,6 +15,7 @@ enum Field { F_RU = 0, F_RV, F_RW, F_RO, F_TT, F_BX, F_BY, F_BZ, NFIELD };
 struct Params {
     int npx, npy, npz;          // interior sizes
     int lmag, lrot, izc, itc, ibc;
+    int tile_y;                 // rows per stage-kernel tile (4 or 8)
     long long plane;            // npx*npy
     long long fsize;            // plane*(npz+2): one field incl. z ghost planes
     double hx, h2x, hy, h2y, hz, h2z, c13, c23, c43;
@@ -41,7 +42,8 @@ struct Metrics {
 // Per-step device scalars.  The reductions keep order-preserving integer
 // keys of the doubles so that min/max can use 64-bit integer atomics.
 struct DevScalars {
-    double dt, umach;
+    double dt, umach;                 // for the next step (from the last reduction)
+    double dt_used, umach_used;       // of the step that last completed
     double cflmin[4];
     double timc, timi, timt;
     unsigned long long key_mach;      // max of OGAMMA*UU/TT
@@ -62,12 +64,11 @@ __host__ __device__ inline double dunkey(unsigned long long k) {
 // Launchers (mhd_kernels.hip).
 hipError_t launch_stage(int mode, const Params &p, const Metrics &m, const double *qin, double *qout,
                         double *z, double *fout, DevScalars *sc, hipStream_t s);
-hipError_t launch_bcon(const Params &p, const Metrics &m, const double *qin, double *qout,
-                       hipStream_t s);
-hipError_t launch_cfl(const Params &p, const Metrics &m, const double *q, DevScalars *sc,
-                      hipStream_t s);
+hipError_t launch_bcon(const Params &p, const Metrics &m, const double *qin, double *qout, DevScalars *sc,
+                       bool cfl, hipStream_t s);
+hipError_t launch_cfl(const Params &p, const Metrics &m, const double *q, DevScalars *sc, hipStream_t s);
+hipError_t launch_cfl_final(const Params &p, DevScalars *sc, bool clock, hipStream_t s);
 hipError_t launch_pack(const Params &p, const double *padded, double *field, hipStream_t s);
-hipError_t launch_unpack(const Params &p, const double *field, double *padded, hipStream_t s);
-hipError_t launch_clock(DevScalars *sc, hipStream_t s);
+hipError_t launch_unpack(const Params &p, const double *field, double *padded, int mode, hipStream_t s);
 
 }  // namespace rmhd

@@ -2,18 +2,30 @@
 //
 // K2 k_stage : right-hand side of FLUXES (3dmhdsub.f:1178-2637) fused with one
 //              Wray RK3 stage update of STEP (3dmhdsub.f:926-969, 977-1067,
-//              1075-1165), one launch per stage.
-// K3 k_bcon  : boundary planes of BCON (3dmhdsub.f:2751-2958).
-// K1 k_cfl   : pointwise CFL limits and maximum Mach number with exact
-//              min/max reductions (3dmhdsub.f:741-873).
+//              1075-1165), one launch per stage.  Stage 3 also reduces the
+//              CFL limits of the new state for the next step (sub:741-873).
+// K3 k_bcon  : boundary planes of BCON (3dmhdsub.f:2751-2958); after stage 3
+//              it adds the boundary planes' CFL contributions.
+// K1 k_cfl   : stand-alone CFL / Mach reduction for the first step after an
+//              upload (sub:741-873).
 // K4 k_pack / k_unpack : conversion between the reference's padded arrays
 //              and the resident layout, with the periodic ghost semantics of
 //              COMM_MPI (3dmhdsub.f:3131-3138).
 //
-// Resident layout: field f of a register is a contiguous block of
-// (npz+2) planes of npx*npy doubles, x fastest.  Plane 0 and npz+1 are the
-// top/bottom z ghost planes, planes 1..npz the reference's K=2..NPZ+1.
-// There are no x/y ghost cells: neighbours wrap periodically.
+// Resident layout: field f of a register is a contiguous block of (npz+2)
+// planes of npx*npy doubles, x fastest.  Plane 0 and npz+1 are the top and
+// bottom z ghost planes, planes 1..npz the reference's K=2..NPZ+1.  There are
+// no x/y ghost cells: neighbours wrap periodically.
+//
+// Stage kernel organisation.  A workgroup owns a 64 x TY column tile and
+// marches down a chunk of planes.  LDS holds the raw fields of planes k and
+// k+1 with a one-point halo (double buffer) and a three-plane ring of the
+// velocities u,v,w (needed at (i+-1,k+-1) and (j+-1,k+-1) by the cross terms
+// of the viscous stress).  Plane k+2 is loaded into registers while plane k
+// is computed, and the Z register of the RK3 scheme is prefetched the same
+// way, so each global load has a whole plane of arithmetic to hide behind.
+// Outputs are written with non-temporal stores so that L2 keeps the halo
+// rows that neighbouring tiles re-read.
 //
 // Arithmetic: every expression keeps the reference's left-to-right
 // association and the file is compiled with -ffp-contract=off, so results
@@ -23,36 +35,30 @@
 
 namespace rmhd {
 
-constexpr int TX = 64;              // interior columns per tile (one wavefront row)
-constexpr int TY = 4;               // interior rows per tile
-constexpr int NT = TX * TY;         // threads per workgroup
-constexpr int SW = TX + 2;          // LDS row stride (1-point halo each side)
-constexpr int SH = TY + 2;
-constexpr int SP = SW * SH;         // points per LDS plane
-constexpr int NHALO = SP - NT;      // halo points, one per thread for t < NHALO
-static_assert(NHALO <= NT, "one halo point per thread at most");
+constexpr int TX = 64;  // interior columns per tile (one wavefront row)
 
-// LDS slots of the current plane.
-enum { C_RU = 0, C_RV, C_RW, C_TT, C_P, C_BX, C_BY, C_BZ };
+template <int TY_>
+struct Tile {
+    static constexpr int TY = TY_;
+    static constexpr int NT = TX * TY;      // threads per workgroup
+    static constexpr int SW = TX + 2;       // LDS row stride (1-point halo each side)
+    static constexpr int SH = TY + 2;
+    static constexpr int SP = SW * SH;      // points per LDS plane
+    static constexpr int NHALO = SP - NT;   // halo points, one per thread for t < NHALO
+    static_assert(NHALO <= NT, "one halo point per thread at most");
+};
+
+// Raw-field slots of the LDS planes (resident field order).
+enum { S_RU = 0, S_RV, S_RW, S_RO, S_TT, S_BX, S_BY, S_BZ };
 
 struct Raw {
-    double ru, rv, rw, ro, tt, bx, by, bz;
+    double v[NFIELD];
 };
 
 template <bool MAG>
 __device__ __forceinline__ void load_raw(const double *__restrict__ q, long long fs, long long off, Raw &r) {
-    r.ru = q[off];
-    r.rv = q[fs + off];
-    r.rw = q[2 * fs + off];
-    r.ro = q[3 * fs + off];
-    r.tt = q[4 * fs + off];
-    if (MAG) {
-        r.bx = q[5 * fs + off];
-        r.by = q[6 * fs + off];
-        r.bz = q[7 * fs + off];
-    } else {
-        r.bx = r.by = r.bz = 0.0;
-    }
+#pragma unroll
+    for (int f = 0; f < (MAG ? 8 : 5); ++f) r.v[f] = q[f * fs + off];
 }
 
 __device__ __forceinline__ int wrapi(int x, int n) {
@@ -60,26 +66,106 @@ __device__ __forceinline__ int wrapi(int x, int n) {
     return r < 0 ? r + n : r;
 }
 
-// XCD-aware block order: the hardware deals consecutive workgroup ids to the
-// 8 XCDs in turn; give each XCD a contiguous run of tiles so that tiles that
-// share halo rows share an L2.
+// XCD-aware block order.  The hardware deals consecutive workgroup ids to
+// the 8 XCDs in turn; remap so that each XCD receives a contiguous run of
+// tiles (tiles that share halo rows then share an L2).  A bijection for any
+// grid size.
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
-    if ((nb & 7) != 0) return bid;
-    return (bid & 7) * (nb >> 3) + (bid >> 3);
+    const int x = bid & 7, local = bid >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return x * q + (x < r ? x : r) + local;
 }
 
-// MODE 0: write the right-hand side F to fout.  MODE 1..3: RK3 stage.
-template <bool MAG, int MODE>
-__global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
-                                              const double *__restrict__ qin,
-                                              double *__restrict__ qout,
-                                              double *__restrict__ zr,
-                                              double *__restrict__ fout,
-                                              const DevScalars *__restrict__ sc,
-                                              int kchunk) {
-    constexpr int NC = MAG ? 8 : 5;
-    __shared__ double s_cur[NC][SP];     // rho u, rho v, rho w, T, P (, B) of plane k
-    __shared__ double s_ring[3][3][SP];  // u, v, w of planes k-1, k, k+1 (slot = plane % 3)
+// Dynamic LDS bytes of a stage workgroup.
+template <int TY, bool MAG>
+constexpr int stage_lds_bytes() {
+    return (2 * (MAG ? 8 : 5) + 9) * Tile<TY>::SP * (int)sizeof(double);
+}
+
+// CFL limits of one point (sub:750-856).  mach = OGAMMA*UU/TT before the
+// square root; w1..w4 the four pointwise limits.
+template <bool MAG>
+__device__ __forceinline__ void cfl_point(const Params &p, const double *q, double ddx, double ddy, double ddz,
+                                          double rkap, double &mach, double &w1, double &w2, double &w3,
+                                          double &w4) {
+    const double ru = q[S_RU], rv = q[S_RV], rw = q[S_RW], ro = q[S_RO], tt = q[S_TT];
+    const double r = 1.0 / ro;
+    double uu = (ru * ru + rv * rv + rw * rw) * (r * r);
+    const double mm = p.ogamma * uu / tt;
+    if (mm > mach || !(mm == mm)) mach = mm;  // NaN propagates into the max
+    if (MAG) {
+        const double bx = q[S_BX], by = q[S_BY], bz = q[S_BZ];
+        const double b2 = bx * bx + by * by + bz * bz;
+        uu = uu + p.gamma * tt + b2 * p.obeta / ro + 2.0 * sqrt(uu * (p.gamma * tt + b2 * p.obeta / ro));
+    } else {
+        uu = uu + p.gamma * tt + 2.0 * sqrt(uu * p.gamma * tt);
+    }
+    double dd = fmin(fmin(ddx, ddy), ddz);
+    const double a1 = dd / sqrt(uu);
+    const double a2 = 0.5 * dd * dd * p.repr * p.cv * ro / rkap;
+    const double a3 = 0.375 * dd * dd * p.re * ro;
+    w1 = (a1 < w1 || !(a1 == a1)) ? a1 : w1;
+    w2 = (a2 < w2 || !(a2 == a2)) ? a2 : w2;
+    w3 = (a3 < w3 || !(a3 == a3)) ? a3 : w3;
+    if (MAG) w4 = fmin(w4, 0.5 * dd * dd * p.rm);
+}
+
+// Workgroup reduction of the CFL accumulators and one set of 64-bit atomics
+// on order-preserving keys (min/max are exact, so the order is irrelevant).
+// NaN keys sort above +Inf: they win the max and, through the key of -NaN
+// (below -Inf), the min, so a non-finite state cannot go unnoticed.
+__device__ void cfl_reduce(double mach, double w1, double w2, double w3, double w4, int lmag,
+                           DevScalars *__restrict__ sc) {
+    __shared__ unsigned long long red[16][5];
+    unsigned long long km = dkey(mach), k1 = dkey(w1), k2 = dkey(w2), k3 = dkey(w3), k4 = dkey(w4);
+    // a NaN limit becomes -NaN for the min reduction
+    if (!(w1 == w1)) k1 = 0;
+    if (!(w2 == w2)) k2 = 0;
+    if (!(w3 == w3)) k3 = 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o;
+        o = __shfl_xor(km, off, 64); km = o > km ? o : km;
+        o = __shfl_xor(k1, off, 64); k1 = o < k1 ? o : k1;
+        o = __shfl_xor(k2, off, 64); k2 = o < k2 ? o : k2;
+        o = __shfl_xor(k3, off, 64); k3 = o < k3 ? o : k3;
+        o = __shfl_xor(k4, off, 64); k4 = o < k4 ? o : k4;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) {
+        red[wv][0] = km; red[wv][1] = k1; red[wv][2] = k2; red[wv][3] = k3; red[wv][4] = k4;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 1; v < nw; ++v) {
+            km = red[v][0] > km ? red[v][0] : km;
+            k1 = red[v][1] < k1 ? red[v][1] : k1;
+            k2 = red[v][2] < k2 ? red[v][2] : k2;
+            k3 = red[v][3] < k3 ? red[v][3] : k3;
+            k4 = red[v][4] < k4 ? red[v][4] : k4;
+        }
+        atomicMax(&sc->key_mach, km);
+        atomicMin(&sc->key_min[0], k1);
+        atomicMin(&sc->key_min[1], k2);
+        atomicMin(&sc->key_min[2], k3);
+        if (lmag) atomicMin(&sc->key_min[3], k4);
+    }
+}
+
+// MODE 0: write the right-hand side F to fout.  MODE 1..3: RK3 stage s.
+template <int TY, bool MAG, int MODE>
+__global__ __launch_bounds__(TX *TY) void k_stage(const Params p, const Metrics m,
+                                                  const double *__restrict__ qin,
+                                                  double *__restrict__ qout,
+                                                  double *__restrict__ zr,
+                                                  double *__restrict__ fout,
+                                                  DevScalars *__restrict__ sc,
+                                                  int kchunk) {
+    using T = Tile<TY>;
+    constexpr int SW = T::SW, SP = T::SP;
+    constexpr int NF = MAG ? 8 : 5;
+    extern __shared__ double lds[];
+    double *const s_cur = lds;                  // [2][NF][SP]: raw planes k, k+1 (by parity)
+    double *const s_ring = lds + 2 * NF * SP;   // [3][3][SP]: u, v, w of planes k-1, k, k+1
 
     const int t = threadIdx.x;
     const int npx = p.npx, npy = p.npy, npz = p.npz;
@@ -103,10 +189,10 @@ __global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
     // halo point (t < NHALO): top row, bottom row, left column, right column
     int hpt = -1;
     long long hoff = 0;
-    if (t < NHALO) {
+    if (t < T::NHALO) {
         int hlx, hly;
         if (t < SW) { hlx = t; hly = 0; }
-        else if (t < 2 * SW) { hlx = t - SW; hly = SH - 1; }
+        else if (t < 2 * SW) { hlx = t - SW; hly = T::SH - 1; }
         else if (t < 2 * SW + TY) { hlx = 0; hly = t - 2 * SW + 1; }
         else { hlx = SW - 1; hly = t - 2 * SW - TY + 1; }
         hpt = hly * SW + hlx;
@@ -114,95 +200,102 @@ __global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
     }
     const bool has_h = hpt >= 0;
 
-    auto put_cur = [&](int pt, const Raw &r) {
-        s_cur[C_RU][pt] = r.ru;
-        s_cur[C_RV][pt] = r.rv;
-        s_cur[C_RW][pt] = r.rw;
-        s_cur[C_TT][pt] = r.tt;
-        s_cur[C_P][pt] = r.ro * r.tt;           // pressure (sub:1389)
-        if (MAG) {
-            s_cur[C_BX][pt] = r.bx;
-            s_cur[C_BY][pt] = r.by;
-            s_cur[C_BZ][pt] = r.bz;
-        }
+    auto cur = [&](int k, int f, int pt) -> double & { return s_cur[((k & 1) * NF + f) * SP + pt]; };
+    auto ring = [&](int k, int c, int pt) -> double & { return s_ring[((k % 3) * 3 + c) * SP + pt]; };
+    auto put_cur = [&](int k, int pt, const Raw &r) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) cur(k, f, pt) = r.v[f];
     };
-    auto put_ring = [&](int slot, int pt, const Raw &r) {
-        const double ri = 1.0 / r.ro;          // sub:1390
-        s_ring[slot][0][pt] = r.ru * ri;        // sub:1414-1424
-        s_ring[slot][1][pt] = r.rv * ri;
-        s_ring[slot][2][pt] = r.rw * ri;
+    auto put_ring = [&](int k, int pt, const Raw &r) {
+        const double ri = 1.0 / r.v[S_RO];     // sub:1390
+        ring(k, 0, pt) = r.v[S_RU] * ri;        // sub:1414-1424
+        ring(k, 1, pt) = r.v[S_RV] * ri;
+        ring(k, 2, pt) = r.v[S_RW] * ri;
     };
 
-    Raw rm_, rc, rn_o, rn_h, rp_o, rp_h;
-    // warm-up: planes kbeg-1, kbeg, kbeg+1
-    {
-        Raw a;
-        load_raw<MAG>(qin, fs, (long long)(kbeg - 1) * pl + ooff, rm_);
-        put_ring((kbeg - 1) % 3, opt, rm_);
+    Raw rm_, rp_o, rp_h;
+    // warm-up: planes kbeg-1 (own values + ring), kbeg and kbeg+1 (LDS + ring)
+    load_raw<MAG>(qin, fs, (long long)(kbeg - 1) * pl + ooff, rm_);
+    put_ring(kbeg - 1, opt, rm_);
+    if (has_h) {
+        load_raw<MAG>(qin, fs, (long long)(kbeg - 1) * pl + hoff, rp_h);
+        put_ring(kbeg - 1, hpt, rp_h);
+    }
+#pragma unroll 1
+    for (int kk = kbeg; kk <= kbeg + 1; ++kk) {
+        load_raw<MAG>(qin, fs, (long long)kk * pl + ooff, rp_o);
+        put_cur(kk, opt, rp_o);
+        put_ring(kk, opt, rp_o);
         if (has_h) {
-            load_raw<MAG>(qin, fs, (long long)(kbeg - 1) * pl + hoff, a);
-            put_ring((kbeg - 1) % 3, hpt, a);
-        }
-        load_raw<MAG>(qin, fs, (long long)kbeg * pl + ooff, rc);
-        put_cur(opt, rc);
-        put_ring(kbeg % 3, opt, rc);
-        if (has_h) {
-            load_raw<MAG>(qin, fs, (long long)kbeg * pl + hoff, a);
-            put_cur(hpt, a);
-            put_ring(kbeg % 3, hpt, a);
-        }
-        load_raw<MAG>(qin, fs, (long long)(kbeg + 1) * pl + ooff, rn_o);
-        put_ring((kbeg + 1) % 3, opt, rn_o);
-        if (has_h) {
-            load_raw<MAG>(qin, fs, (long long)(kbeg + 1) * pl + hoff, rn_h);
-            put_ring((kbeg + 1) % 3, hpt, rn_h);
+            load_raw<MAG>(qin, fs, (long long)kk * pl + hoff, rp_h);
+            put_cur(kk, hpt, rp_h);
+            put_ring(kk, hpt, rp_h);
         }
     }
 
     const double HX = p.hx, H2X = p.h2x, HY = p.hy, H2Y = p.h2y, HZ = p.hz, H2Z = p.h2z;
     const double C43 = p.c43, OCV = p.ocv, ORE = p.ore, ORM = p.orm, OBETA = p.obeta;
-    // per-column metrics
-    const double dx = m.dxxdx[wrapi(gi, npx) + 1], d2x = m.d2xxdx2[wrapi(gi, npx) + 1];
-    const double dy = m.dyydy[wrapi(gj, npy) + 1], d2y = m.d2yydy2[wrapi(gj, npy) + 1];
+    const int gix = wrapi(gi, npx) + 1, gjy = wrapi(gj, npy) + 1;
+    const double dx = m.dxxdx[gix], d2x = m.d2xxdx2[gix];
+    const double dy = m.dyydy[gjy], d2y = m.d2yydy2[gjy];
     const double tmpy = HY * dy;
     const double tmpy1 = HY * d2y;
     const double tmpy2 = H2Y * dy * dy;
-    double dt = 0.0, coefg = 0.0, coefz = 0.0;
+    double coefg = 0.0, coefz = 0.0;
     if (MODE > 0) {
-        dt = sc->dt;
+        const double dt = sc->dt;
         coefg = (MODE == 1 ? p.gam1 : MODE == 2 ? p.gam2 : p.gam3) * dt;
         coefz = (MODE == 1 ? p.zeta1 : p.zeta2) * dt;
     }
+    // CFL accumulators of the new state (stage 3 only)
+    double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
+    const double c_ddx = MODE == 3 ? m.ddx[gix] : 0.0, c_ddy = MODE == 3 ? m.ddy[gjy] : 0.0;
 
+#pragma unroll 1
     for (int k = kbeg; k <= kend; ++k) {
         const bool pre = k < kend;
+        const long long o = (long long)k * pl + ooff;
         if (pre) {
             load_raw<MAG>(qin, fs, (long long)(k + 2) * pl + ooff, rp_o);
             if (has_h) load_raw<MAG>(qin, fs, (long long)(k + 2) * pl + hoff, rp_h);
         }
+        double zk[8];
+        if (MODE >= 2 && valid) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) zk[f] = zr[f * fs + o];
+        }
         __syncthreads();
         if (valid) {
-            const int sm = (k + 2) % 3, s0 = k % 3, sp = (k + 1) % 3;
             const bool top = (k == 1), bot = (k == npz);
             const double dz = m.dzzdz[k], d2z = m.d2zzdz2[k];
             const double rk = m.rkapa[k], dk = m.dkapa[k];
             const double tmpz = HZ * dz;
             const double tmpz1 = HZ * d2z;
             const double tmpz2 = H2Z * dz * dz;
-#define CUR(f, d) s_cur[f][opt + (d)]
-#define RNG(s, c, d) s_ring[s][c][opt + (d)]
-            const double ro0 = rc.ro, tt0 = rc.tt;
+#define CUR(f, d) cur(k, f, opt + (d))
+#define NXT(f) cur(k + 1, f, opt)
+#define RNG(kp, c, d) ring(kp, c, opt + (d))
+            double q0[8];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) q0[f] = CUR(f, 0);
+            const double ro0 = q0[S_RO], tt0 = q0[S_TT];
             const double rinv = 1.0 / ro0;
-            const double u0 = RNG(s0, 0, 0), v0 = RNG(s0, 1, 0), w0 = RNG(s0, 2, 0);
-            const double um = RNG(sm, 0, 0), vm = RNG(sm, 1, 0), wm = RNG(sm, 2, 0);
-            const double up = RNG(sp, 0, 0), vp = RNG(sp, 1, 0), wp = RNG(sp, 2, 0);
-            const double uxp = RNG(s0, 0, 1), uxm = RNG(s0, 0, -1);
-            const double uyp = RNG(s0, 0, SW), uym = RNG(s0, 0, -SW);
-            const double vxp = RNG(s0, 1, 1), vxm = RNG(s0, 1, -1);
-            const double vyp = RNG(s0, 1, SW), vym = RNG(s0, 1, -SW);
-            const double wxp = RNG(s0, 2, 1), wxm = RNG(s0, 2, -1);
-            const double wyp = RNG(s0, 2, SW), wym = RNG(s0, 2, -SW);
-            const double pm = rm_.ro * rm_.tt, pp = rn_o.ro * rn_o.tt;
+            const double u0 = RNG(k, 0, 0), v0 = RNG(k, 1, 0), w0 = RNG(k, 2, 0);
+            const double um = RNG(k - 1, 0, 0), vm = RNG(k - 1, 1, 0), wm = RNG(k - 1, 2, 0);
+            const double up = RNG(k + 1, 0, 0), vp = RNG(k + 1, 1, 0), wp = RNG(k + 1, 2, 0);
+            const double uxp = RNG(k, 0, 1), uxm = RNG(k, 0, -1);
+            const double uyp = RNG(k, 0, SW), uym = RNG(k, 0, -SW);
+            const double vxp = RNG(k, 1, 1), vxm = RNG(k, 1, -1);
+            const double vyp = RNG(k, 1, SW), vym = RNG(k, 1, -SW);
+            const double wxp = RNG(k, 2, 1), wxm = RNG(k, 2, -1);
+            const double wyp = RNG(k, 2, SW), wym = RNG(k, 2, -SW);
+            // own column, planes k-1 and k+1
+            const double ru_m = rm_.v[S_RU], rv_m = rm_.v[S_RV], rw_m = rm_.v[S_RW], tt_m = rm_.v[S_TT];
+            const double ru_p = NXT(S_RU), rv_p = NXT(S_RV), rw_p = NXT(S_RW), tt_p = NXT(S_TT);
+            const double pm = rm_.v[S_RO] * tt_m, pp = NXT(S_RO) * tt_p;
+            // pressure RO*TT at the in-plane neighbours (sub:1389)
+            const double pxm = CUR(S_RO, -1) * CUR(S_TT, -1), pxp = CUR(S_RO, 1) * CUR(S_TT, 1);
+            const double pym = CUR(S_RO, -SW) * CUR(S_TT, -SW), pyp = CUR(S_RO, SW) * CUR(S_TT, SW);
 
             // extra own-column values for the one-sided forms on the boundary planes
             double rwk2 = 0.0, wk2 = 0.0, bxk2 = 0.0, bxk3 = 0.0, byk2 = 0.0, byk3 = 0.0;
@@ -221,35 +314,35 @@ __global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
             }
 
             // ---- continuity (sub:1252-1295)
-            double fr = (CUR(C_RU, -1) - CUR(C_RU, 1)) * HX * dx;
-            fr = fr - (CUR(C_RV, SW) - CUR(C_RV, -SW)) * tmpy;
+            double fr = (CUR(S_RU, -1) - CUR(S_RU, 1)) * HX * dx;
+            fr = fr - (CUR(S_RV, SW) - CUR(S_RV, -SW)) * tmpy;
             double wz;
-            if (top) wz = (4.0 * rn_o.rw - rwk2) * tmpz;
-            else if (bot) wz = (rwk2 - 4.0 * rm_.rw) * tmpz;
-            else wz = (rn_o.rw - rm_.rw) * tmpz;
+            if (top) wz = (4.0 * rw_p - rwk2) * tmpz;
+            else if (bot) wz = (rwk2 - 4.0 * rw_m) * tmpz;
+            else wz = (rw_p - rw_m) * tmpz;
             fr = fr - wz;
 
             // ---- buoyancy and pressure gradient (sub:1379, 1401-1403)
             double fw = p.grav * ro0;
             fw = fw - (pp - pm) * tmpz;
-            double fv = (CUR(C_P, -SW) - CUR(C_P, SW)) * tmpy;
-            double fu = (CUR(C_P, -1) - CUR(C_P, 1)) * HX * dx;
+            double fv = (pym - pyp) * tmpy;
+            double fu = (pxm - pxp) * HX * dx;
 
             // ---- momentum advection (sub:1440-1481)
-            fu = fu - (CUR(C_RU, 1) * uxp - CUR(C_RU, -1) * uxm) * HX * dx;
-            fu = fu - (CUR(C_RU, SW) * vyp - CUR(C_RU, -SW) * vym) * tmpy;
-            fu = fu - (rn_o.ru * wp - rm_.ru * wm) * tmpz;
-            fv = fv - (CUR(C_RV, 1) * uxp - CUR(C_RV, -1) * uxm) * HX * dx;
-            fv = fv - (CUR(C_RV, SW) * vyp - CUR(C_RV, -SW) * vym) * tmpy;
-            fv = fv - (rn_o.rv * wp - rm_.rv * wm) * tmpz;
-            fw = fw - (CUR(C_RW, 1) * uxp - CUR(C_RW, -1) * uxm) * HX * dx;
-            fw = fw - (CUR(C_RW, SW) * vyp - CUR(C_RW, -SW) * vym) * tmpy;
-            fw = fw - (rn_o.rw * wp - rm_.rw * wm) * tmpz;
+            fu = fu - (CUR(S_RU, 1) * uxp - CUR(S_RU, -1) * uxm) * HX * dx;
+            fu = fu - (CUR(S_RU, SW) * vyp - CUR(S_RU, -SW) * vym) * tmpy;
+            fu = fu - (ru_p * wp - ru_m * wm) * tmpz;
+            fv = fv - (CUR(S_RV, 1) * uxp - CUR(S_RV, -1) * uxm) * HX * dx;
+            fv = fv - (CUR(S_RV, SW) * vyp - CUR(S_RV, -SW) * vym) * tmpy;
+            fv = fv - (rv_p * wp - rv_m * wm) * tmpz;
+            fw = fw - (CUR(S_RW, 1) * uxp - CUR(S_RW, -1) * uxm) * HX * dx;
+            fw = fw - (CUR(S_RW, SW) * vyp - CUR(S_RW, -SW) * vym) * tmpy;
+            fw = fw - (rw_p * wp - rw_m * wm) * tmpz;
 
             // ---- temperature advection (sub:1500-1505)
-            const double txp = CUR(C_TT, 1), txm = CUR(C_TT, -1);
-            const double typ = CUR(C_TT, SW), tym = CUR(C_TT, -SW);
-            const double tzp = rn_o.tt, tzm = rm_.tt;
+            const double txp = CUR(S_TT, 1), txm = CUR(S_TT, -1);
+            const double typ = CUR(S_TT, SW), tym = CUR(S_TT, -SW);
+            const double tzp = tt_p, tzm = tt_m;
             double ft = -(u0 * (txp - txm) * HX * dx);
             ft = ft - v0 * (typ - tym) * tmpy;
             ft = ft - w0 * (tzp - tzm) * tmpz;
@@ -282,11 +375,11 @@ __global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
             ft = ft + (C43 * dxu * dxu + dxv * dxv + dxw * dxw) * rinv * tv;
             ft = ft - OCV * dxu * tt0;
             {
-                const double dxv_jp = (RNG(s0, 1, SW + 1) - RNG(s0, 1, SW - 1)) * HX * dx;
-                const double dxv_jm = (RNG(s0, 1, -SW + 1) - RNG(s0, 1, -SW - 1)) * HX * dx;
+                const double dxv_jp = (RNG(k, 1, SW + 1) - RNG(k, 1, SW - 1)) * HX * dx;
+                const double dxv_jm = (RNG(k, 1, -SW + 1) - RNG(k, 1, -SW - 1)) * HX * dx;
                 fu = fu + tcr * (dxv_jp - dxv_jm) * HY * dy;
-                const double dxu_jp = (RNG(s0, 0, SW + 1) - RNG(s0, 0, SW - 1)) * HX * dx;
-                const double dxu_jm = (RNG(s0, 0, -SW + 1) - RNG(s0, 0, -SW - 1)) * HX * dx;
+                const double dxu_jp = (RNG(k, 0, SW + 1) - RNG(k, 0, SW - 1)) * HX * dx;
+                const double dxu_jm = (RNG(k, 0, -SW + 1) - RNG(k, 0, -SW - 1)) * HX * dx;
                 fv = fv + tcr * (dxu_jp - dxu_jm) * HY * dy;
             }
             const double dyu = (uyp - uym) * HY * dy;
@@ -298,11 +391,11 @@ __global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
             ft = ft + (dzw * dzw - dxu * dzw - dxu * dyv - dyv * dzw) * rinv * tv43;
             ft = ft - OCV * dzw * tt0;
             {
-                const double dzw_ip = (RNG(sp, 2, 1) - RNG(sm, 2, 1)) * HZ * dz;
-                const double dzw_im = (RNG(sp, 2, -1) - RNG(sm, 2, -1)) * HZ * dz;
+                const double dzw_ip = (RNG(k + 1, 2, 1) - RNG(k - 1, 2, 1)) * HZ * dz;
+                const double dzw_im = (RNG(k + 1, 2, -1) - RNG(k - 1, 2, -1)) * HZ * dz;
                 fu = fu + tcr * (dzw_ip - dzw_im) * HX * dx;
-                const double dzw_jp = (RNG(sp, 2, SW) - RNG(sm, 2, SW)) * HZ * dz;
-                const double dzw_jm = (RNG(sp, 2, -SW) - RNG(sm, 2, -SW)) * HZ * dz;
+                const double dzw_jp = (RNG(k + 1, 2, SW) - RNG(k - 1, 2, SW)) * HZ * dz;
+                const double dzw_jm = (RNG(k + 1, 2, -SW) - RNG(k - 1, 2, -SW)) * HZ * dz;
                 fv = fv + tcr * (dzw_jp - dzw_jm) * HY * dy;
             }
             const double dzu = (up - um) * HZ * dz;
@@ -310,33 +403,33 @@ __global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
             const double dyw = (wyp - wym) * HY * dy;
             ft = ft + (dzu * dzu + dzv * dzv + dyw * dyw + 2.0 * dzv * dyw) * rinv * tv;
             {
-                const double dzu_ip = (RNG(sp, 0, 1) - RNG(sm, 0, 1)) * HZ * dz;
-                const double dzu_im = (RNG(sp, 0, -1) - RNG(sm, 0, -1)) * HZ * dz;
-                const double dzv_jp = (RNG(sp, 1, SW) - RNG(sm, 1, SW)) * HZ * dz;
-                const double dzv_jm = (RNG(sp, 1, -SW) - RNG(sm, 1, -SW)) * HZ * dz;
+                const double dzu_ip = (RNG(k + 1, 0, 1) - RNG(k - 1, 0, 1)) * HZ * dz;
+                const double dzu_im = (RNG(k + 1, 0, -1) - RNG(k - 1, 0, -1)) * HZ * dz;
+                const double dzv_jp = (RNG(k + 1, 1, SW) - RNG(k - 1, 1, SW)) * HZ * dz;
+                const double dzv_jm = (RNG(k + 1, 1, -SW) - RNG(k - 1, 1, -SW)) * HZ * dz;
                 fw = fw + tcr * ((dzu_ip - dzu_im) * HX * dx + (dzv_jp - dzv_jm) * HY * dy);
             }
             ft = ft + 2.0 * dzu * dxw * rinv * tv;
 
             // ---- Coriolis (sub:2106-2125)
             if (p.lrot) {
-                fu = fu + p.omz * rc.rv;
-                fv = fv - p.omz * rc.ru + p.omx * rc.rw;
-                fw = fw - p.omx * rc.rv;
+                fu = fu + p.omz * q0[S_RV];
+                fv = fv - p.omz * q0[S_RU] + p.omx * q0[S_RW];
+                fw = fw - p.omx * q0[S_RV];
             }
 
             // ---- induction, Lorentz force, Ohmic heating, resistive diffusion (sub:2129-2634)
             double w1 = 0.0, w2 = 0.0, w3 = 0.0;
             if (MAG) {
-                const double bx0 = rc.bx, by0 = rc.by, bz0 = rc.bz;
-                const double bxm = rm_.bx, bym = rm_.by, bzm = rm_.bz;
-                const double bxp = rn_o.bx, byp = rn_o.by, bzp = rn_o.bz;
-                const double bxxp = CUR(C_BX, 1), bxxm = CUR(C_BX, -1);
-                const double bxyp = CUR(C_BX, SW), bxym = CUR(C_BX, -SW);
-                const double byxp = CUR(C_BY, 1), byxm = CUR(C_BY, -1);
-                const double byyp = CUR(C_BY, SW), byym = CUR(C_BY, -SW);
-                const double bzxp = CUR(C_BZ, 1), bzxm = CUR(C_BZ, -1);
-                const double bzyp = CUR(C_BZ, SW), bzym = CUR(C_BZ, -SW);
+                const double bx0 = q0[S_BX], by0 = q0[S_BY], bz0 = q0[S_BZ];
+                const double bxm = rm_.v[S_BX], bym = rm_.v[S_BY], bzm = rm_.v[S_BZ];
+                const double bxp = NXT(S_BX), byp = NXT(S_BY), bzp = NXT(S_BZ);
+                const double bxxp = CUR(S_BX, 1), bxxm = CUR(S_BX, -1);
+                const double bxyp = CUR(S_BX, SW), bxym = CUR(S_BX, -SW);
+                const double byxp = CUR(S_BY, 1), byxm = CUR(S_BY, -1);
+                const double byyp = CUR(S_BY, SW), byym = CUR(S_BY, -SW);
+                const double bzxp = CUR(S_BZ, 1), bzxm = CUR(S_BZ, -1);
+                const double bzyp = CUR(S_BZ, SW), bzym = CUR(S_BZ, -SW);
                 const double tm = p.t_ohm;
 
                 w2 = bx0 * dxv - by0 * dxu;
@@ -399,9 +492,9 @@ __global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
 
                 const double dzby = (byp - bym) * HZ * dz;
                 const double dybz = (bzyp - bzym) * HY * dy;
-                const double cur = dzby - dybz;
-                fv = fv + OBETA * bz0 * cur;
-                fw = fw - OBETA * by0 * cur;
+                const double jx = dzby - dybz;
+                fv = fv + OBETA * bz0 * jx;
+                fw = fw - OBETA * by0 * jx;
                 ft = ft + (dzby * dzby + dybz * dybz - 2.0 * dzby * dybz) * rinv * tm;
                 w2 = w2 - w0 * dzby;
                 w3 = w3 - v0 * dybz;
@@ -414,179 +507,155 @@ __global__ __launch_bounds__(NT) void k_stage(const Params p, const Metrics m,
                 w3 = w3 + ORM * d2zbz;
             }
 #undef CUR
+#undef NXT
 #undef RNG
             // ---- output
-            const long long o = (long long)k * pl + ooff;
-            const double fv8[8] = {fu, fv, fw, fr, ft, w1, w2, w3};
+            const double F[8] = {fu, fv, fw, fr, ft, w1, w2, w3};
             if (MODE == 0) {
 #pragma unroll
-                for (int f = 0; f < (MAG ? 8 : 5); ++f) fout[f * fs + o] = fv8[f];
+                for (int f = 0; f < NF; ++f) fout[f * fs + o] = F[f];
             } else {
-                const double q8[8] = {rc.ru, rc.rv, rc.rw, rc.ro, rc.tt, rc.bx, rc.by, rc.bz};
+                double qn[8];
 #pragma unroll
-                for (int f = 0; f < (MAG ? 8 : 5); ++f) {
-                    const double zin = (MODE == 1) ? q8[f] : zr[f * fs + o];
-                    const double qn = zin + coefg * fv8[f];
-                    qout[f * fs + o] = qn;
-                    if (MODE < 3) zr[f * fs + o] = qn + coefz * fv8[f];
+                for (int f = 0; f < NF; ++f) {
+                    const double zin = (MODE == 1) ? q0[f] : zk[f];
+                    qn[f] = zin + coefg * F[f];
+                    __builtin_nontemporal_store(qn[f], &qout[f * fs + o]);
+                    if (MODE < 3) __builtin_nontemporal_store(qn[f] + coefz * F[f], &zr[f * fs + o]);
                 }
+                if (MODE == 3 && !top && !bot)
+                    cfl_point<MAG>(p, qn, c_ddx, c_ddy, m.ddz[k], rk, c_mach, c_w1, c_w2, c_w3, c_w4);
             }
+            // own raw values of plane k become the k-1 values of the next plane
+#pragma unroll
+            for (int f = 0; f < NF; ++f) rm_.v[f] = q0[f];
         }
         __syncthreads();
-        // advance the window: k+1 becomes current, k+2 enters the ring
-        rm_ = rc;
-        rc = rn_o;
-        put_cur(opt, rn_o);
-        if (has_h) put_cur(hpt, rn_h);
+        // plane k+2 replaces plane k in LDS and enters the ring
         if (pre) {
-            put_ring((k + 2) % 3, opt, rp_o);
-            if (has_h) put_ring((k + 2) % 3, hpt, rp_h);
-            rn_o = rp_o;
-            rn_h = rp_h;
+            put_cur(k + 2, opt, rp_o);
+            put_ring(k + 2, opt, rp_o);
+            if (has_h) {
+                put_cur(k + 2, hpt, rp_h);
+                put_ring(k + 2, hpt, rp_h);
+            }
         }
     }
+    if (MODE == 3) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
 }
 
 // BCON on the two boundary planes of the freshly written register, and the
 // in-place 1/rho of the untouched z ghost planes that FLUXES performs
 // (sub:1390) so that the resident ghosts follow the reference exactly.
-template <bool MAG>
+// With CFL set (after stage 3), the boundary planes' CFL limits of the final
+// state are added to the reduction.
+template <bool MAG, bool CFL>
 __global__ __launch_bounds__(256) void k_bcon(const Params p, const Metrics m, const double *__restrict__ qin,
-                                              double *__restrict__ q) {
+                                              double *__restrict__ q, DevScalars *__restrict__ sc) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= p.npx * p.npy) return;
-    const long long fs = p.fsize, pl = p.plane;
-    const int nz = p.npz;
-    const double C13 = p.c13, C23 = p.c23, C43 = p.c43;
-    double *RU = q, *RV = q + fs, *RW = q + 2 * fs, *RO = q + 3 * fs, *TT = q + 4 * fs, *BZ = q + 7 * fs;
-    // top (sub:2771-2868)
-    {
-        const long long a = pl + idx, b = 2 * pl + idx, c = 3 * pl + idx;
-        RU[a] = RO[a] * (C43 * RU[b] / RO[b] - C13 * RU[c] / RO[c]);
-        RV[a] = RO[a] * (C43 * RV[b] / RO[b] - C13 * RV[c] / RO[c]);
-        RW[a] = 0.0;
-        if (p.itc == 0) {
-            TT[a] = 1.0;
-        } else {
-            const double dz = p.dzf / m.dzzdz[1];
-            TT[a] = C43 * TT[b] - C13 * TT[c] - C23 * dz * p.theta;
+    double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
+    if (idx < p.npx * p.npy) {
+        const long long fs = p.fsize, pl = p.plane;
+        const int nz = p.npz;
+        const double C13 = p.c13, C23 = p.c23, C43 = p.c43;
+        double *RU = q, *RV = q + fs, *RW = q + 2 * fs, *RO = q + 3 * fs, *TT = q + 4 * fs, *BZ = q + 7 * fs;
+        // top (sub:2771-2868)
+        {
+            const long long a = pl + idx, b = 2 * pl + idx, c = 3 * pl + idx;
+            RU[a] = RO[a] * (C43 * RU[b] / RO[b] - C13 * RU[c] / RO[c]);
+            RV[a] = RO[a] * (C43 * RV[b] / RO[b] - C13 * RV[c] / RO[c]);
+            RW[a] = 0.0;
+            if (p.itc == 0) {
+                TT[a] = 1.0;
+            } else {
+                const double dz = p.dzf / m.dzzdz[1];
+                TT[a] = C43 * TT[b] - C13 * TT[c] - C23 * dz * p.theta;
+            }
+            if (MAG && p.ibc == 0) BZ[a] = 0.0;
         }
-        if (MAG && p.ibc == 0) BZ[a] = 0.0;
-    }
-    // bottom (sub:2873-2955)
-    {
-        const long long a = (long long)nz * pl + idx, b = (long long)(nz - 1) * pl + idx,
-                        c = (long long)(nz - 2) * pl + idx;
-        RU[a] = RO[a] * (C43 * RU[b] / RO[b] - C13 * RU[c] / RO[c]);
-        RV[a] = RO[a] * (C43 * RV[b] / RO[b] - C13 * RV[c] / RO[c]);
-        RW[a] = 0.0;
-        if (p.izc == 0) {
-            TT[a] = p.tb;
-        } else {
-            const double dz = p.dzf / m.dzzdz[nz];
-            TT[a] = C43 * TT[b] - C13 * TT[c] + C23 * dz * p.theta / m.rkapa[nz];
+        // bottom (sub:2873-2955)
+        {
+            const long long a = (long long)nz * pl + idx, b = (long long)(nz - 1) * pl + idx,
+                            c = (long long)(nz - 2) * pl + idx;
+            RU[a] = RO[a] * (C43 * RU[b] / RO[b] - C13 * RU[c] / RO[c]);
+            RV[a] = RO[a] * (C43 * RV[b] / RO[b] - C13 * RV[c] / RO[c]);
+            RW[a] = 0.0;
+            if (p.izc == 0) {
+                TT[a] = p.tb;
+            } else {
+                const double dz = p.dzf / m.dzzdz[nz];
+                TT[a] = C43 * TT[b] - C13 * TT[c] + C23 * dz * p.theta / m.rkapa[nz];
+            }
+            if (MAG && p.ibc == 0) BZ[a] = 0.0;
         }
-        if (MAG && p.ibc == 0) BZ[a] = 0.0;
+        // z ghost planes of RO flip to 1/RO once per FLUXES call
+        const double *ROi = qin + 3 * fs;
+        RO[idx] = 1.0 / ROi[idx];
+        RO[(long long)(nz + 1) * pl + idx] = 1.0 / ROi[(long long)(nz + 1) * pl + idx];
+        if (CFL) {
+            const int j = idx / p.npx, i = idx - j * p.npx;
+            for (int kb = 0; kb < 2; ++kb) {
+                const int k = kb ? nz : 1;
+                const long long o = (long long)k * pl + idx;
+                double v[8];
+#pragma unroll
+                for (int f = 0; f < (MAG ? 8 : 5); ++f) v[f] = q[f * fs + o];
+                cfl_point<MAG>(p, v, m.ddx[i + 1], m.ddy[j + 1], m.ddz[k], m.rkapa[k], c_mach, c_w1, c_w2,
+                               c_w3, c_w4);
+            }
+        }
     }
-    // z ghost planes of RO flip to 1/RO once per FLUXES call
-    const double *ROi = qin + 3 * fs;
-    RO[idx] = 1.0 / ROi[idx];
-    RO[(long long)(nz + 1) * pl + idx] = 1.0 / ROi[(long long)(nz + 1) * pl + idx];
+    if (CFL) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
 }
 
-// wave64 + workgroup reductions of the CFL limits (sub:747-872)
+// Stand-alone CFL reduction over planes 1..npz (first step after an upload).
 template <bool MAG>
 __global__ __launch_bounds__(256) void k_cfl(const Params p, const Metrics m, const double *__restrict__ q,
                                              DevScalars *__restrict__ sc) {
     const long long fs = p.fsize, pl = p.plane;
     const long long n = (long long)p.npz * pl;
     double mach = 0.0, w1 = INFINITY, w2 = INFINITY, w3 = INFINITY, w4 = INFINITY;
-    int bad = 0;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long long)gridDim.x * blockDim.x) {
-        const long long o = pl + idx;      // planes 1..npz
+        const long long o = pl + idx;
         const int k = (int)(idx / pl) + 1;
         const long long r2 = idx - (long long)(k - 1) * pl;
         const int j = (int)(r2 / p.npx), i = (int)(r2 - (long long)j * p.npx);
-        const double ru = q[o], rv = q[fs + o], rw = q[2 * fs + o], ro = q[3 * fs + o], tt = q[4 * fs + o];
-        const double r = 1.0 / ro;
-        double uu = (ru * ru + rv * rv + rw * rw) * (r * r);
-        const double mm = p.ogamma * uu / tt;
-        if (mm > mach) mach = mm;
-        if (MAG) {
-            const double bx = q[5 * fs + o], by = q[6 * fs + o], bz = q[7 * fs + o];
-            const double b2 = bx * bx + by * by + bz * bz;
-            uu = uu + p.gamma * tt + b2 * p.obeta / ro + 2.0 * sqrt(uu * (p.gamma * tt + b2 * p.obeta / ro));
-        } else {
-            uu = uu + p.gamma * tt + 2.0 * sqrt(uu * p.gamma * tt);
-        }
-        double dd = m.ddx[i + 1];
-        dd = fmin(dd, m.ddy[j + 1]);
-        dd = fmin(dd, m.ddz[k]);
-        const double a1 = dd / sqrt(uu);
-        const double a2 = 0.5 * dd * dd * p.repr * p.cv * ro / m.rkapa[k];
-        const double a3 = 0.375 * dd * dd * p.re * ro;
-        if (!(a1 == a1) || !(a2 == a2) || !(a3 == a3) || !(mm == mm)) bad = 1;
-        w1 = fmin(w1, a1);
-        w2 = fmin(w2, a2);
-        w3 = fmin(w3, a3);
-        if (MAG) w4 = fmin(w4, 0.5 * dd * dd * p.rm);
+        double v[8];
+#pragma unroll
+        for (int f = 0; f < (MAG ? 8 : 5); ++f) v[f] = q[f * fs + o];
+        cfl_point<MAG>(p, v, m.ddx[i + 1], m.ddy[j + 1], m.ddz[k], m.rkapa[k], mach, w1, w2, w3, w4);
     }
-    // wave64 butterfly
-    for (int off = 32; off > 0; off >>= 1) {
-        mach = fmax(mach, __shfl_xor(mach, off, 64));
-        w1 = fmin(w1, __shfl_xor(w1, off, 64));
-        w2 = fmin(w2, __shfl_xor(w2, off, 64));
-        w3 = fmin(w3, __shfl_xor(w3, off, 64));
-        w4 = fmin(w4, __shfl_xor(w4, off, 64));
-        bad |= __shfl_xor(bad, off, 64);
-    }
-    __shared__ double red[4][5];
-    __shared__ int rbad[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
-        red[wv][0] = mach; red[wv][1] = w1; red[wv][2] = w2; red[wv][3] = w3; red[wv][4] = w4;
-        rbad[wv] = bad;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int v = 1; v < (int)(blockDim.x >> 6); ++v) {
-            mach = fmax(mach, red[v][0]);
-            w1 = fmin(w1, red[v][1]);
-            w2 = fmin(w2, red[v][2]);
-            w3 = fmin(w3, red[v][3]);
-            w4 = fmin(w4, red[v][4]);
-            bad |= rbad[v];
-        }
-        atomicMax(&sc->key_mach, dkey(mach));
-        atomicMin(&sc->key_min[0], dkey(w1));
-        atomicMin(&sc->key_min[1], dkey(w2));
-        atomicMin(&sc->key_min[2], dkey(w3));
-        if (MAG) atomicMin(&sc->key_min[3], dkey(w4));
-        if (bad) atomicOr(&sc->nonfinite, 1);
-    }
+    cfl_reduce(mach, w1, w2, w3, w4, p.lmag, sc);
 }
 
-// DT = SF*MIN(...), UMACH = SQRT(MAX(...)) (sub:761, 868-872); re-arms the keys.
-__global__ void k_cfl_final(DevScalars *sc, double sf, int lmag) {
+// End of a reduction: DT = SF*MIN(...), UMACH = SQRT(MAX(...)) (sub:761,
+// 868-872), then re-arm the keys.  With clock set this also closes the step
+// that just ran: NIT, TIMC, TIMT (sub:1171-1173) with the DT it used, which
+// is kept as dt_used / umach_used.
+__global__ void k_cfl_final(DevScalars *sc, double sf, int lmag, int clock) {
+    if (clock) {
+        sc->nit = sc->nit + 1;
+        sc->timc = sc->timc + sc->dt;
+        sc->timt = sc->timi + sc->timc;
+        sc->dt_used = sc->dt;
+        sc->umach_used = sc->umach;
+    }
     const double mach = dunkey(sc->key_mach);
     double mn = INFINITY;
+    bool bad = !(mach == mach);
     for (int n = 0; n < 4; ++n) {
         sc->cflmin[n] = dunkey(sc->key_min[n]);
-        if (n < 3 || lmag) mn = fmin(mn, sc->cflmin[n]);
+        if (n < 3 || lmag) {
+            mn = fmin(mn, sc->cflmin[n]);
+            bad = bad || !(sc->cflmin[n] == sc->cflmin[n]);
+        }
     }
     sc->umach = sqrt(mach);
     sc->dt = sf * mn;
-    if (!(sc->dt == sc->dt) || !(sc->umach == sc->umach) || sc->dt == INFINITY) sc->nonfinite |= 2;
+    if (bad || sc->dt == INFINITY) sc->nonfinite |= 1;
     sc->key_mach = dkey(0.0);
     for (int n = 0; n < 4; ++n) sc->key_min[n] = dkey(INFINITY);
-}
-
-// NIT, TIMC, TIMT bookkeeping at the end of STEP (sub:1171-1173).
-__global__ void k_clock(DevScalars *sc) {
-    sc->nit = sc->nit + 1;
-    sc->timc = sc->timc + sc->dt;
-    sc->timt = sc->timi + sc->timc;
 }
 
 // padded Fortran (npx+2, npy+2, npz+2) -> resident field (x/y ghosts dropped)
@@ -625,8 +694,8 @@ __global__ void k_unpack(const Params p, const double *__restrict__ field, doubl
 // ---------------------------------------------------------------------------
 // launchers
 
-static int chunk_for(const Params &p, int *nchunks) {
-    const int ntiles = ((p.npx + TX - 1) / TX) * ((p.npy + TY - 1) / TY);
+static int chunk_for(const Params &p, int ty, int *nchunks) {
+    const int ntiles = ((p.npx + TX - 1) / TX) * ((p.npy + ty - 1) / ty);
     int nc = (2048 + ntiles - 1) / ntiles;            // aim for >= 2048 workgroups
     int maxc = p.npz / 8;                              // but keep >= 8 planes per chunk
     if (maxc < 1) maxc = 1;
@@ -637,37 +706,61 @@ static int chunk_for(const Params &p, int *nchunks) {
     return kc;
 }
 
-hipError_t launch_stage(int mode, const Params &p, const Metrics &m, const double *qin, double *qout,
-                        double *z, double *fout, DevScalars *sc, hipStream_t s) {
-    int nch;
-    const int kc = chunk_for(p, &nch);
-    const int ntiles = ((p.npx + TX - 1) / TX) * ((p.npy + TY - 1) / TY);
-    dim3 grid(ntiles * nch), block(NT);
-#define L(MAG, MODE) hipLaunchKernelGGL((k_stage<MAG, MODE>), grid, block, 0, s, p, m, qin, qout, z, fout, sc, kc)
-    if (p.lmag) {
-        switch (mode) {
-            case 0: L(true, 0); break;
-            case 1: L(true, 1); break;
-            case 2: L(true, 2); break;
-            default: L(true, 3); break;
-        }
-    } else {
-        switch (mode) {
-            case 0: L(false, 0); break;
-            case 1: L(false, 1); break;
-            case 2: L(false, 2); break;
-            default: L(false, 3); break;
-        }
+template <int TY, bool MAG, int MODE>
+static hipError_t launch_stage_t(const Params &p, const Metrics &m, const double *qin, double *qout, double *z,
+                                 double *fout, DevScalars *sc, hipStream_t s) {
+    constexpr int lds = stage_lds_bytes<TY, MAG>();
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<TY, MAG, MODE>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        attr = true;
     }
-#undef L
+    int nch;
+    const int kc = chunk_for(p, TY, &nch);
+    const int ntiles = ((p.npx + TX - 1) / TX) * ((p.npy + TY - 1) / TY);
+    hipLaunchKernelGGL((k_stage<TY, MAG, MODE>), dim3(ntiles * nch), dim3(TX * TY), lds, s, p, m, qin, qout, z,
+                       fout, sc, kc);
     return hipGetLastError();
 }
 
-hipError_t launch_bcon(const Params &p, const Metrics &m, const double *qin, double *qout, hipStream_t s) {
+template <int TY>
+static hipError_t launch_stage_ty(int mode, const Params &p, const Metrics &m, const double *qin, double *qout,
+                                  double *z, double *fout, DevScalars *sc, hipStream_t s) {
+    if (p.lmag) {
+        switch (mode) {
+            case 0: return launch_stage_t<TY, true, 0>(p, m, qin, qout, z, fout, sc, s);
+            case 1: return launch_stage_t<TY, true, 1>(p, m, qin, qout, z, fout, sc, s);
+            case 2: return launch_stage_t<TY, true, 2>(p, m, qin, qout, z, fout, sc, s);
+            default: return launch_stage_t<TY, true, 3>(p, m, qin, qout, z, fout, sc, s);
+        }
+    }
+    switch (mode) {
+        case 0: return launch_stage_t<TY, false, 0>(p, m, qin, qout, z, fout, sc, s);
+        case 1: return launch_stage_t<TY, false, 1>(p, m, qin, qout, z, fout, sc, s);
+        case 2: return launch_stage_t<TY, false, 2>(p, m, qin, qout, z, fout, sc, s);
+        default: return launch_stage_t<TY, false, 3>(p, m, qin, qout, z, fout, sc, s);
+    }
+}
+
+hipError_t launch_stage(int mode, const Params &p, const Metrics &m, const double *qin, double *qout,
+                        double *z, double *fout, DevScalars *sc, hipStream_t s) {
+    if (p.tile_y == 8) return launch_stage_ty<8>(mode, p, m, qin, qout, z, fout, sc, s);
+    return launch_stage_ty<4>(mode, p, m, qin, qout, z, fout, sc, s);
+}
+
+hipError_t launch_bcon(const Params &p, const Metrics &m, const double *qin, double *qout, DevScalars *sc,
+                       bool cfl, hipStream_t s) {
     const int n = p.npx * p.npy;
     dim3 grid((n + 255) / 256), block(256);
-    if (p.lmag) hipLaunchKernelGGL(k_bcon<true>, grid, block, 0, s, p, m, qin, qout);
-    else hipLaunchKernelGGL(k_bcon<false>, grid, block, 0, s, p, m, qin, qout);
+    if (p.lmag) {
+        if (cfl) hipLaunchKernelGGL((k_bcon<true, true>), grid, block, 0, s, p, m, qin, qout, sc);
+        else hipLaunchKernelGGL((k_bcon<true, false>), grid, block, 0, s, p, m, qin, qout, sc);
+    } else {
+        if (cfl) hipLaunchKernelGGL((k_bcon<false, true>), grid, block, 0, s, p, m, qin, qout, sc);
+        else hipLaunchKernelGGL((k_bcon<false, false>), grid, block, 0, s, p, m, qin, qout, sc);
+    }
     return hipGetLastError();
 }
 
@@ -678,12 +771,11 @@ hipError_t launch_cfl(const Params &p, const Metrics &m, const double *q, DevSca
     dim3 grid((unsigned)nb), block(256);
     if (p.lmag) hipLaunchKernelGGL(k_cfl<true>, grid, block, 0, s, p, m, q, sc);
     else hipLaunchKernelGGL(k_cfl<false>, grid, block, 0, s, p, m, q, sc);
-    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(1), 0, s, sc, p.sf, p.lmag);
     return hipGetLastError();
 }
 
-hipError_t launch_clock(DevScalars *sc, hipStream_t s) {
-    hipLaunchKernelGGL(k_clock, dim3(1), dim3(1), 0, s, sc);
+hipError_t launch_cfl_final(const Params &p, DevScalars *sc, bool clock, hipStream_t s) {
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(1), 0, s, sc, p.sf, p.lmag, clock ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -694,19 +786,11 @@ hipError_t launch_pack(const Params &p, const double *padded, double *field, hip
     return hipGetLastError();
 }
 
-hipError_t launch_unpack(const Params &p, const double *field, double *padded, hipStream_t s) {
+hipError_t launch_unpack(const Params &p, const double *field, double *padded, int mode, hipStream_t s) {
     const long long n = (long long)(p.npx + 2) * (p.npy + 2) * (p.npz + 2);
     long long nb = (n + 255) / 256;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, s, p, field, padded, 1);
-    return hipGetLastError();
-}
-
-hipError_t launch_unpack_rhs(const Params &p, const double *field, double *padded, hipStream_t s) {
-    const long long n = (long long)(p.npx + 2) * (p.npy + 2) * (p.npz + 2);
-    long long nb = (n + 255) / 256;
-    if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, s, p, field, padded, 0);
+    hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, s, p, field, padded, mode);
     return hipGetLastError();
 }
 
