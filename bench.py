@@ -145,6 +145,14 @@ def main():
                    "sample": f"C oracle (restated reference step), 128x128x64 cut of the workload, "
                              f"{n} steps, {s:.3f} s/step, 1 thread"}
 
+    # HBM bytes per stage launch from the committed rocprofv3 --pmc summary of
+    # this workload (FETCH_SIZE / WRITE_SIZE, calibrated on a plain copy)
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "k_stage_512x512x256_fp64_traffic.json")
+    if (npx, npy, npz) == (512, 512, 256) and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["bytes_per_launch_mean"]
+
     out = {
         "metric": "RK3 zone-updates/s (plain FP64 MHD step, 512x512x256) and %HBM peak",
         "value": zu,
@@ -162,7 +170,8 @@ def main():
                    "npx": npx, "npy": npy, "npz": npz, "parallelism": "single GPU, whole box resident"},
         "hbm_frac_step": zu * BYTES_PER_ZONE_STEP / HBM_PEAK,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": stage_bytes,
                      "kernel": "k_stage (fused RHS + RK3 stage), mean of the 3 stage launches",
                      "ms_stage": kt["stage"], "ms_cfl": kt["cfl"], "ms_bcon": kt["bcon"],
                      "ms_step_events": kt["step"]},
