@@ -1,15 +1,21 @@
 """Throughput of the explicit compressible-MHD RK3 step on one MI355X.
 
-Workload (default): 512x512x256 box, MHD with a 45-degree tilted rotation
+Headline workload: 512x512x256 box, MHD with a 45-degree tilted rotation
 axis (latitude f-plane), m=1 polytrope with density contrast 30, Ra=1e6,
 Pr=1, Pm=2, Ta=1e6, arctan-stretched z grid (max/min spacing 3), seeded
 uniform random T perturbation 1e-4 and a seed field from a random vector
-potential in modes |k|<=8 at amplitude 1e-3.  Plain FP64 (the drop-in form,
-bit-identical to the reference).  Synthetic initial data; adaptive dt.
+potential in modes |k|<=8 at amplitude 1e-3, 500 RK3 steps with adaptive dt.
+`value` is the mixed-precision form (FP32 perturbations about the FP64
+hydrostatic background, FP64 accumulation); the plain FP64 form (the
+drop-in, bit-identical to the reference) is timed on the same box beside it.
+
+Accuracy of the mixed form: a 256x256x128 box (contrast 10, Ra 1e5, Pr 1,
+Pm 1, Ta 1e4 vertical axis, tanh z grid, seed field |k|<=4 at 1e-5) is run
+1000 steps in both forms and the per-field errors are reported.
 
 A step is one full RK3 step (CFL reduction + 3 stages + boundary planes).
 The timed region is K steps back to back with the inputs resident in HBM.
-Prints one JSON line (see DESIGN.md for the field definitions).
+Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -24,33 +30,74 @@ import sys
 import tempfile
 import time
 
-import numpy as np
-
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
-HBM_PEAK = 8.0e12            # MI355X HBM3E, bytes/s
-BYTES_PER_ZONE_STEP = 640.0  # 8 fields x 10 accesses x 8 B (q, Z registers; one kernel per stage)
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# algorithmic HBM bytes per zone per RK3 step (8 fields, one kernel per stage)
+BYTES_PER_ZONE_STEP = {
+    "fp64": 640.0,   # q read + Z read/write + q write: 10 accesses x 8 B
+    "mixed": 384.0,  # per stage h read + D read + h write + D write: 4 x 4 B
+}
+KERNEL = {"fp64": "k_stage (fused RHS + RK3 stage, FP64)",
+          "mixed": "k_stage_m (fused FP32 perturbation RHS + FP64 RK3 accumulation)"}
+TRAFFIC = {"fp64": "k_stage_512x512x256_fp64_traffic.json", "mixed": "k_stage_m_512x512x256_mixed_traffic.json"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def wl():
+    return importlib.import_module("rast-3dmhd_amd.workload")
+
+
 def workload(npx, npy, npz, seed=20260):
-    cfg = importlib.import_module("rast-3dmhd_amd.config")
-    ics = importlib.import_module("rast-3dmhd_amd.ics")
-    ph = cfg.physics(contrast=30.0, ra=1e6, pr=1.0, pm=2.0, ta=1e6, tilt_deg=45.0)
-    par = {k: cfg.f32(v) for k, v in ph.items()}
-    mt = cfg.metrics(npx, npy, npz, xmax=par[17], ymax=par[18], zmax=par[19], ngrid=1)
-    t, r = cfg.polytrope(mt["zee"], par[3], par[4])
-    meta = cfg.constants(npx, npy, npz, par, lmag=True, lrot=True, izcon=1, itcon=0, tb=float(t[-1]))
-    meta.update(mt)
-    st = ics.static_state(npx, npy, npz, t, r, ampt=1e-4, lmag=True, seed=seed, fast_noise=True)
-    st["bx"], st["by"], st["bz"] = ics.vector_potential_field(meta, kmax=8, amp=1e-3, seed=seed)
-    for k in ("ru", "rv", "rw", "ro", "tt"):
-        ics.periodic_fill(st[k])
-    return meta, st
+    return wl().convection_box(npx, npy, npz, seed=seed)
+
+
+def timed(pkg, meta, st, precision, steps, warmup):
+    box = pkg.Box(meta, precision=precision)
+    try:
+        box.upload(st)
+        log(f"{precision}: warmup {warmup}")
+        box.steps(warmup)
+        log(f"{precision}: timing {steps} steps")
+        t0 = time.perf_counter()
+        dt, umach = box.steps(steps)
+        wall = time.perf_counter() - t0
+        # per-kernel durations from HIP events on the library's stream
+        kt = box.time_steps(min(steps, 5))
+    finally:
+        box.close()
+    return {"wall": wall, "ms_per_step": 1e3 * wall / steps, "kt": kt, "dt": dt, "umach": umach}
+
+
+def roofline(precision, zones, kt, shape):
+    stage_bytes = zones * BYTES_PER_ZONE_STEP[precision] / 3.0
+    achieved = stage_bytes / (kt["stage"] * 1e-3)
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", TRAFFIC[precision])
+    if shape == (512, 512, 256) and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["bytes_per_launch_mean"]
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "algorithmic_bytes_per_launch": stage_bytes,
+            "kernel": KERNEL[precision] + ", mean of the 3 stage launches",
+            "ms_stage": kt["stage"], "ms_cfl": kt["cfl"], "ms_bcon": kt["bcon"], "ms_step_events": kt["step"]}
+
+
+def accuracy(npx=256, npy=256, npz=128, steps=1000):
+    acc = importlib.import_module("rast-3dmhd_amd.accuracy")
+    r = acc.compare(npx, npy, npz, steps, log=log)
+    e = r["errors"]
+    return {"box": r["box"], "steps": steps, "workload": r["workload"],
+            "rel_l2": e["rel_l2"], "max_rel_l2": e["max_rel_l2"], "velocity_rel_l2": e["velocity"],
+            "entropy_rel_fluct_l2": e["entropy"], "rho_rel_fluct_l2": e["fluct_l2"]["ro"],
+            "tt_rel_fluct_l2": e["fluct_l2"]["tt"], "b_rel_l2": e.get("b_field"),
+            "divb_max_mixed": e["divb_max"]["test"], "divb_max_fp64": e["divb_max"]["ref"],
+            "timc_rel_diff": r["timc_rel_diff"],
+            "ms_step_fp64": r["fp64"]["ms_step_events"], "ms_step_mixed": r["mixed"]["ms_step_events"]}
 
 
 def cpu_baseline_reference(exe, ranks, n1=1, n2=3, timeout=900):
@@ -90,11 +137,13 @@ def cpu_baseline_port(seconds=15.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--npx", type=int, default=512)
     ap.add_argument("--npy", type=int, default=512)
     ap.add_argument("--npz", type=int, default=256)
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--accuracy-steps", type=int, default=1000, help="0 skips the mixed-vs-FP64 error run")
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--ranks", type=int, default=16)
     args = ap.parse_args()
@@ -105,38 +154,33 @@ def main():
 
     pkg = importlib.import_module("rast-3dmhd_amd")
     pkg.load()
-    npx, npy, npz = args.npx, args.npy, args.npz
-    zones = npx * npy * npz
-    log(f"building workload {npx}x{npy}x{npz}")
-    meta, st = workload(npx, npy, npz)
-    box = pkg.Box(meta)
-    box.upload(st)
+    shape = (args.npx, args.npy, args.npz)
+    zones = args.npx * args.npy * args.npz
+    log(f"building workload {shape}")
+    meta, st = workload(*shape)
+    forms = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    res = {p: timed(pkg, meta, st, p, args.steps, args.warmup) for p in forms}
     del st
-    log("warmup")
-    box.steps(args.warmup)
-    log(f"timing {args.steps} steps")
-    t0 = time.perf_counter()
-    dt, umach = box.steps(args.steps)
-    wall = time.perf_counter() - t0
-    ms_step = 1e3 * wall / args.steps
-    zu = zones * args.steps / wall
-    # per-kernel durations from HIP events on the library's stream
-    kt = box.time_steps(min(args.steps, 5))
-    stage_bytes = zones * BYTES_PER_ZONE_STEP / 3.0
-    achieved = stage_bytes / (kt["stage"] * 1e-3)
-    box.close()
+    head = forms[0]
+    zu = {p: zones * args.steps / r["wall"] for p, r in res.items()}
+
+    acc = None
+    if args.accuracy_steps > 0 and args.precision == "both":
+        log(f"accuracy: mixed vs fp64, 256x256x128, {args.accuracy_steps} steps")
+        acc = accuracy(steps=args.accuracy_steps)
 
     cpu = None
     if args.cpu_baseline != "none":
         exe = os.path.join(REPO, "oracle", "_ref", "bench512", "3dmhd.exe")
-        use_ref = args.cpu_baseline in ("auto", "reference") and os.path.exists(exe) and (npx, npy, npz) == (512, 512, 256)
+        use_ref = args.cpu_baseline in ("auto", "reference") and os.path.exists(exe) and shape == (512, 512, 256)
         if use_ref:
             try:
                 s_per_step = cpu_baseline_reference(exe, args.ranks)
                 cpu = {"value": zones / s_per_step, "unit": "zone-updates/s", "cores": args.ranks,
                        "kind": "reference",
-                       "sample": f"reference Fortran/MPI build, {npx}x{npy}x{npz}, {args.ranks} ranks "
-                                 f"(NPEY=8,NPEZ=2), (wall(3 steps)-wall(1 step))/2 = {s_per_step:.2f} s/step"}
+                       "sample": f"reference Fortran/MPI build (plain FP64), {args.npx}x{args.npy}x{args.npz}, "
+                                 f"{args.ranks} ranks (NPEY=8,NPEZ=2), (wall(3 steps)-wall(1 step))/2 "
+                                 f"= {s_per_step:.2f} s/step"}
             except Exception as e:  # fall back to the port
                 log(f"reference baseline failed: {e}")
         if cpu is None and args.cpu_baseline != "reference":
@@ -145,39 +189,36 @@ def main():
                    "sample": f"C oracle (restated reference step), 128x128x64 cut of the workload, "
                              f"{n} steps, {s:.3f} s/step, 1 thread"}
 
-    # HBM bytes per stage launch from the committed rocprofv3 --pmc summary of
-    # this workload (FETCH_SIZE / WRITE_SIZE, calibrated on a plain copy)
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", "k_stage_512x512x256_fp64_traffic.json")
-    if (npx, npy, npz) == (512, 512, 256) and os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f)["bytes_per_launch_mean"]
-
+    forms_out = {}
+    for p, r in res.items():
+        forms_out[p] = {"value": zu[p], "unit": "zone-updates/s", "ms_per_step": r["ms_per_step"],
+                        "hbm_frac_step": zu[p] * BYTES_PER_ZONE_STEP[p] / HBM_PEAK,
+                        "bytes_per_zone_step": BYTES_PER_ZONE_STEP[p],
+                        "roofline": roofline(p, zones, r["kt"], shape), "last_dt": r["dt"], "last_umach": r["umach"]}
     out = {
-        "metric": "RK3 zone-updates/s (plain FP64 MHD step, 512x512x256) and %HBM peak",
-        "value": zu,
+        "metric": "RK3 zone-updates/s + %HBM peak; mixed vs FP64 at stated error",
+        "value": zu[head],
         "unit": "zone-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": res[head]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head == "mixed" else "f64",
         "data": "synthetic (polytrope + seeded T noise + random-vector-potential seed field)",
-        "config": {"workload": f"{npx}x{npy}x{npz} MHD + 45deg rotation, contrast 30, Ra 1e6, Pm 2, Ta 1e6, plain FP64",
-                   "npx": npx, "npy": npy, "npz": npz, "parallelism": "single GPU, whole box resident"},
-        "hbm_frac_step": zu * BYTES_PER_ZONE_STEP / HBM_PEAK,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": stage_bytes,
-                     "kernel": "k_stage (fused RHS + RK3 stage), mean of the 3 stage launches",
-                     "ms_stage": kt["stage"], "ms_cfl": kt["cfl"], "ms_bcon": kt["bcon"],
-                     "ms_step_events": kt["step"]},
+        "config": {"workload": f"{args.npx}x{args.npy}x{args.npz} MHD + 45deg rotation, contrast 30, Ra 1e6, "
+                               f"Pm 2, Ta 1e6, {args.steps} RK3 steps, "
+                               + ("mixed precision" if head == "mixed" else "plain FP64"),
+                   "npx": args.npx, "npy": args.npy, "npz": args.npz,
+                   "precision": head, "parallelism": "single GPU, whole box resident"},
+        "hbm_frac_step": forms_out[head]["hbm_frac_step"],
+        "roofline": forms_out[head]["roofline"],
         "cpu_baseline": cpu,
-        "last_dt": dt,
-        "last_umach": umach,
+        "forms": forms_out,
+        "mixed_speedup_vs_fp64": (zu["mixed"] / zu["fp64"]) if len(res) == 2 else None,
+        "mixed_accuracy": acc,
     }
     print(json.dumps(out))
 

@@ -46,7 +46,7 @@ typedef struct mhd_desc {
     int npx, npy, npz;          /* interior points; z includes both boundary planes */
     int lmag, lrot;             /* LMAG, LROT */
     int izcon, itcon, ibcon;    /* bottom T (0 fixed, 1 flux), top T (0/1), B (0: BZ=0) */
-    int reserved;
+    int precision;              /* MHD_FP64 (bit-identical to the reference) or MHD_MIXED */
     double hx, h2x, hy, h2y, hz, h2z, c13, c23, c43;   /* /GRID/ */
     double gam1, gam2, gam3, zeta1, zeta2;              /* /RUNGKU/ */
     double cv, ocv, ore, re, repr, theta, grav, sf, gamma; /* /CPAR/ */
@@ -56,6 +56,14 @@ typedef struct mhd_desc {
     double tp;                                          /* /CPER/ TP (must be 0) */
     double dzf;             /* float32 1.0E00/FLOAT(NPZ-1) widened (BCON, sub:2856/2929) */
 } mhd_desc;
+
+/* Precision of the step (mhd_desc.precision). */
+enum {
+    MHD_FP64 = 0,   /* plain FP64, bit-identical to the reference build */
+    MHD_MIXED = 1   /* FP32 perturbations about an FP64 hydrostatic background,
+                       FP64 RK3 accumulation, boundary planes and dt reduction
+                       (no counterpart in the reference) */
+};
 
 typedef struct mhd_ctx mhd_t;
 
@@ -70,6 +78,11 @@ int mhd_set_metrics(mhd_t *h,
                     const double *dyydy, const double *d2yydy2, const double *ddy,
                     const double *dzzdz, const double *d2zzdz2, const double *ddz,
                     const double *rkapa, const double *dkapa);
+
+/* MHD_MIXED only (no counterpart in the reference): the hydrostatic
+ * background rho0(z), T0(z) about which perturbations are carried, padded
+ * length NZ.  Call after mhd_set_metrics and before mhd_upload. */
+int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0);
 
 /* Uploads the prognostic arrays RU,RV,RW,RO,TT[,BX,BY,BZ] of /BIG/.  The
  * x/y ghosts are ignored (the library wraps periodically, which is what

@@ -21,8 +21,10 @@ FIELDS = ("ru", "rv", "rw", "ro", "tt", "bx", "by", "bz")
 RHS = ("fu", "fv", "fw", "fr", "ft", "ww1", "ww2", "ww3")
 
 MHD_OK, MHD_ERR_ARG, MHD_ERR_SCOPE, MHD_ERR_HIP, MHD_ERR_STATE, MHD_ERR_NONFINITE = range(6)
+MHD_FP64, MHD_MIXED = 0, 1
+PRECISIONS = {"fp64": MHD_FP64, "mixed": MHD_MIXED}
 
-_DESC_INTS = ["npx", "npy", "npz", "lmag", "lrot", "izcon", "itcon", "ibcon", "reserved"]
+_DESC_INTS = ["npx", "npy", "npz", "lmag", "lrot", "izcon", "itcon", "ibcon", "precision"]
 _DESC_DBLS = ["hx", "h2x", "hy", "h2y", "hz", "h2z", "c13", "c23", "c43",
               "gam1", "gam2", "gam3", "zeta1", "zeta2",
               "cv", "ocv", "ore", "re", "repr", "theta", "grav", "sf", "gamma",
@@ -57,6 +59,7 @@ def load(path: str | None = None):
     lib = C.CDLL(path)
     lib.mhd_create.argtypes = [C.POINTER(MhdDesc), C.POINTER(C.c_void_p)]
     lib.mhd_set_metrics.argtypes = [C.c_void_p] + [_P] * 11
+    lib.mhd_set_background.argtypes = [C.c_void_p, _P, _P]
     lib.mhd_upload.argtypes = [C.c_void_p] + [_P] * 8
     lib.mhd_download.argtypes = [C.c_void_p] + [_P] * 8
     lib.mhd_step.argtypes = [C.c_void_p, _P, _P]
@@ -69,17 +72,18 @@ def load(path: str | None = None):
     lib.mhd_last_error.restype = C.c_char_p
     lib.mhd_destroy.argtypes = [C.c_void_p]
     lib.mhd_destroy.restype = None
-    for fn in ("mhd_create", "mhd_set_metrics", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
+    for fn in ("mhd_create", "mhd_set_metrics", "mhd_set_background", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
                "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps"):
         getattr(lib, fn).restype = C.c_int
     _lib = lib
     return lib
 
 
-def make_desc(meta: dict) -> MhdDesc:
+def make_desc(meta: dict, precision: str = "fp64") -> MhdDesc:
     """mhd_desc from a dict of COMMON scalars (names as in the reference,
     lower case; npx/npy/npz are interior sizes)."""
     d = MhdDesc()
+    d.precision = PRECISIONS[precision]
     d.npx, d.npy, d.npz = int(meta["npx"]), int(meta["npy"]), int(meta["npz"])
     d.lmag, d.lrot = int(meta.get("lmag", 0)), int(meta.get("lrot", 0))
     d.izcon = int(meta.get("izc", meta.get("izcon", 0)))
@@ -95,12 +99,20 @@ def _ptr(a):
 
 
 class Box:
-    """One resident box on the current GPU, driven through the C-ABI."""
+    """One resident box on the current GPU, driven through the C-ABI.
 
-    def __init__(self, meta: dict):
+    precision="fp64" is the drop-in form (bit-identical to the reference);
+    "mixed" carries FP32 perturbations about an FP64 background (rho0(z),
+    T0(z)).  For the mixed form, set_background() fixes the background; if
+    it was not called, the first upload() uses the horizontal means of the
+    uploaded RO and TT planes."""
+
+    def __init__(self, meta: dict, precision: str = "fp64"):
         self.lib = load()
         self.meta = meta
-        self.desc = make_desc(meta)
+        self.precision = precision
+        self.desc = make_desc(meta, precision)
+        self._have_bg = False
         self.shape = (self.desc.npx + 2, self.desc.npy + 2, self.desc.npz + 2)
         self.lmag = bool(self.desc.lmag)
         h = C.c_void_p()
@@ -121,7 +133,21 @@ class Box:
             raise ValueError(f"array shape {a.shape}, expected {self.shape}")
         return np.asfortranarray(a)
 
+    def set_background(self, ro0, tt0):
+        """FP64 background profiles, arrays of npz+2 plane values (z ghost
+        planes included); mixed form only."""
+        ro0 = np.ascontiguousarray(ro0, dtype=np.float64)
+        tt0 = np.ascontiguousarray(tt0, dtype=np.float64)
+        if ro0.shape != (self.shape[2],) or tt0.shape != (self.shape[2],):
+            raise ValueError(f"background profiles must have {self.shape[2]} planes")
+        self._check(self.lib.mhd_set_background(self.h, _ptr(ro0), _ptr(tt0)))
+        self._have_bg = True
+
     def upload(self, fields: dict):
+        if self.precision == "mixed" and not self._have_bg:
+            ro = np.asarray(fields["ro"], dtype=np.float64)
+            tt = np.asarray(fields["tt"], dtype=np.float64)
+            self.set_background(ro[1:-1, 1:-1, :].mean(axis=(0, 1)), tt[1:-1, 1:-1, :].mean(axis=(0, 1)))
         arrs = [self._padded(fields[k]) if (k in fields and (self.lmag or k not in ("bx", "by", "bz"))) else None
                 for k in FIELDS]
         self._check(self.lib.mhd_upload(self.h, *[_ptr(a) for a in arrs]))

@@ -1,9 +1,11 @@
 // C-ABI of the MI355X compressible-MHD step (declared in include/rast3dmhd.h).
 //
-// Owns the device state of one resident box: two q registers (ping-pong,
-// because a stage reads neighbours of the old state while writing the new
-// one), the Z register of Wray's low-storage RK3, the metrics, and a small
-// block of per-step scalars.  One HIP stream per handle.
+// Owns the device state of one resident box.  FP64 form: two q registers
+// (ping-pong, because a stage reads the old state's neighbours while writing
+// the new one) and the Z register of Wray's low-storage RK3.  Mixed form: two
+// FP32 perturbation registers h (ping-pong) and the FP32 register D, plus the
+// FP64 background.  Both: the metrics and a small block of per-step scalars.
+// One HIP stream per handle.
 
 #include <hip/hip_runtime.h>
 
@@ -13,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "../../include/rast3dmhd.h"
 #include "mhd_kernels.h"
@@ -24,16 +27,23 @@ struct mhd_ctx {
     hipStream_t stream = nullptr;
     Params p{};
     mhd_desc desc{};
-    double *q[2] = {nullptr, nullptr};  // ping-pong state registers
-    double *z = nullptr;                // RK3 Z register
+    int mixed = 0;
+    double *q[2] = {nullptr, nullptr};  // FP64: ping-pong state registers
+    double *z = nullptr;                // FP64: RK3 Z register
+    float *hq[2] = {nullptr, nullptr};  // mixed: FP32 perturbation registers
+    float *dr = nullptr;                // mixed: FP32 increment/residual register
+    double *bgd = nullptr;              // mixed: FP64 background arrays
+    float *bgf = nullptr;               // mixed: FP32 background arrays
+    Background bg{};
     double *f = nullptr;                // right-hand side (mhd_fluxes only)
     double *stage = nullptr;            // one padded field for host transfers
     double *met = nullptr;              // 11 metric arrays
     Metrics m{};
+    std::vector<double> dzzdz, d2zzdz2, rkapa, dkapa;  // host copies for the background
     DevScalars *sc = nullptr;
     int cur = 0;
-    bool have_metrics = false, have_state = false;
-    bool cfl_valid = false;             // sc->dt/umach describe q[cur]
+    bool have_metrics = false, have_state = false, have_bg = false;
+    bool cfl_valid = false;             // sc->dt/umach describe the current register
     char err[512] = {0};
 };
 
@@ -62,7 +72,8 @@ static void free_all(mhd_t *h) {
     if (!h) return;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *bufs[7] = {h->q[0], h->q[1], h->z, h->f, h->stage, h->met, h->sc};
+    void *bufs[12] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->bgd, h->bgf,
+                      h->f, h->stage, h->met, h->sc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -80,9 +91,12 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
         return fail(nullptr, MHD_ERR_SCOPE, "mhd_create: ITCON=%d not supported (0 or 1)", d->itcon);
     if (d->tp != 0.0)
         return fail(nullptr, MHD_ERR_SCOPE, "mhd_create: plume boundary (TP != 0) not supported");
+    if (d->precision != MHD_FP64 && d->precision != MHD_MIXED)
+        return fail(nullptr, MHD_ERR_ARG, "mhd_create: precision %d unknown", d->precision);
     mhd_t *h = new (std::nothrow) mhd_ctx();
     if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_create: out of host memory");
     h->desc = *d;
+    h->mixed = d->precision == MHD_MIXED;
     Params &p = h->p;
     p.npx = d->npx; p.npy = d->npy; p.npz = d->npz;
     p.lmag = d->lmag ? 1 : 0; p.lrot = d->lrot ? 1 : 0;
@@ -106,14 +120,28 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
     p.t_cross = d->c13 * d->ore;           // sub:1930
     p.t_ohm = d->obeta * d->orm * d->ocv;  // sub:2287
     int rc;
-    if (hipGetDevice(&h->dev) != hipSuccess) { rc = fail(nullptr, MHD_ERR_HIP, "mhd_create: no HIP device"); delete h; return rc; }
-    const size_t fbytes = (size_t)p.fsize * sizeof(double);
-    const size_t pbytes = (size_t)(d->npx + 2) * (d->npy + 2) * (d->npz + 2) * sizeof(double);
+    if (hipGetDevice(&h->dev) != hipSuccess) {
+        rc = fail(nullptr, MHD_ERR_HIP, "mhd_create: no HIP device");
+        delete h;
+        return rc;
+    }
     const int nf = p.lmag ? 8 : 5;
+    const size_t pbytes = (size_t)(d->npx + 2) * (d->npy + 2) * (d->npz + 2) * sizeof(double);
+    const size_t nzp = (size_t)d->npz + 2;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&h->q[0], fbytes * nf);
-    if (e == hipSuccess) e = hipMalloc(&h->q[1], fbytes * nf);
-    if (e == hipSuccess) e = hipMalloc(&h->z, fbytes * nf);
+    if (!h->mixed) {
+        const size_t fbytes = (size_t)p.fsize * sizeof(double) * nf;
+        if (e == hipSuccess) e = hipMalloc(&h->q[0], fbytes);
+        if (e == hipSuccess) e = hipMalloc(&h->q[1], fbytes);
+        if (e == hipSuccess) e = hipMalloc(&h->z, fbytes);
+    } else {
+        const size_t fbytes = (size_t)p.fsize * sizeof(float) * nf;
+        if (e == hipSuccess) e = hipMalloc(&h->hq[0], fbytes);
+        if (e == hipSuccess) e = hipMalloc(&h->hq[1], fbytes);
+        if (e == hipSuccess) e = hipMalloc(&h->dr, fbytes);
+        if (e == hipSuccess) e = hipMalloc(&h->bgd, 4 * nzp * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&h->bgf, 5 * nzp * sizeof(float));
+    }
     if (e == hipSuccess) e = hipMalloc(&h->stage, pbytes);
     if (e == hipSuccess) e = hipMalloc(&h->met, sizeof(double) * (3 * (d->npx + 2) + 3 * (d->npy + 2) + 5 * (d->npz + 2)));
     if (e == hipSuccess) e = hipMalloc(&h->sc, sizeof(DevScalars));
@@ -157,14 +185,75 @@ extern "C" int mhd_set_metrics(mhd_t *h, const double *dxxdx, const double *d2xx
         *dst[n] = h->met + off;
         off += len[n];
     }
+    const size_t nz = (size_t)h->p.npz + 2;
+    h->dzzdz.assign(dzzdz, dzzdz + nz);
+    h->d2zzdz2.assign(d2zzdz2, d2zzdz2 + nz);
+    h->rkapa.assign(rkapa, rkapa + nz);
+    h->dkapa.assign(dkapa, dkapa + nz);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->have_metrics = true;
     return MHD_OK;
 }
 
+extern "C" int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0) {
+    if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_set_background: null handle");
+    if (!h->mixed) return fail(h, MHD_ERR_STATE, "mhd_set_background: only for MHD_MIXED handles");
+    if (!h->have_metrics) return fail(h, MHD_ERR_STATE, "mhd_set_background: set the metrics first");
+    if (!ro0 || !tt0) return fail(h, MHD_ERR_ARG, "mhd_set_background: null array");
+    const Params &p = h->p;
+    const int nz = p.npz + 2;
+    std::vector<double> d(4 * (size_t)nz, 0.0);
+    std::vector<float> f(5 * (size_t)nz, 0.0f);
+    double *ro = d.data(), *tt = ro + nz, *fw = tt + nz, *ft = fw + nz;
+    float *ro32 = f.data(), *tt32 = ro32 + nz, *dtz = tt32 + nz, *ri = dtz + nz, *s0 = ri + nz;
+    for (int k = 0; k < nz; ++k) {
+        if (!(ro0[k] > 0.0)) return fail(h, MHD_ERR_ARG, "mhd_set_background: rho0[%d] <= 0", k);
+        ro[k] = ro0[k];
+        tt[k] = tt0[k];
+        ro32[k] = (float)ro0[k];
+        tt32[k] = (float)tt0[k];
+        ri[k] = (float)(1.0 / ro0[k]);
+    }
+    // the background's discrete residuals and coefficients (planes 1..npz)
+    for (int k = 1; k <= p.npz; ++k) {
+        const double dz = h->dzzdz[k], d2z = h->d2zzdz2[k];
+        const double g0 = (tt[k + 1] - tt[k - 1]) * p.hz * dz;
+        const double l0 = (tt[k + 1] - tt[k - 1]) * p.hz * d2z + (tt[k + 1] - 2.0 * tt[k] + tt[k - 1]) * p.h2z * dz * dz;
+        const double sz = p.t_cond * (h->rkapa[k] * l0 + h->dkapa[k] * g0);
+        dtz[k] = (float)g0;
+        s0[k] = (float)sz;
+        ft[k] = sz / ro[k];
+        fw[k] = p.grav * ro[k] - (ro[k + 1] * tt[k + 1] - ro[k - 1] * tt[k - 1]) * p.hz * dz;
+    }
+    HIPCHK(h, hipSetDevice(h->dev));
+    HIPCHK(h, hipMemcpyAsync(h->bgd, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->bgf, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->bg.ro = h->bgd;
+    h->bg.tt = h->bgd + nz;
+    h->bg.fwbg = h->bgd + 2 * nz;
+    h->bg.ftbg = h->bgd + 3 * nz;
+    h->bg.ro32 = h->bgf;
+    h->bg.tt32 = h->bgf + nz;
+    h->bg.dt0dz32 = h->bgf + 2 * nz;
+    h->bg.rinv0_32 = h->bgf + 3 * nz;
+    h->bg.s0_32 = h->bgf + 4 * nz;
+    h->have_bg = true;
+    h->have_state = false;
+    return MHD_OK;
+}
+
+// background column of field f (RO and TT only), or null
+static const double *bgcol(const mhd_t *h, int f) {
+    if (f == F_RO) return h->bg.ro;
+    if (f == F_TT) return h->bg.tt;
+    return nullptr;
+}
+
 extern "C" int mhd_upload(mhd_t *h, const double *ru, const double *rv, const double *rw, const double *ro,
                           const double *tt, const double *bx, const double *by, const double *bz) {
     if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_upload: null handle");
+    if (h->mixed && !h->have_bg) return fail(h, MHD_ERR_STATE, "mhd_upload: set the background first");
     const double *src[8] = {ru, rv, rw, ro, tt, bx, by, bz};
     const int nf = nfields(h);
     const size_t pbytes = (size_t)(h->p.npx + 2) * (h->p.npy + 2) * (h->p.npz + 2) * sizeof(double);
@@ -172,9 +261,15 @@ extern "C" int mhd_upload(mhd_t *h, const double *ru, const double *rv, const do
     for (int f = 0; f < nf; ++f) {
         if (!src[f]) return fail(h, MHD_ERR_ARG, "mhd_upload: field %d is null", f);
         HIPCHK(h, hipMemcpyAsync(h->stage, src[f], pbytes, hipMemcpyHostToDevice, h->stream));
+        const size_t off = (size_t)f * h->p.fsize;
         // both registers carry the z ghost planes, which the stages never write
-        HIPCHK(h, launch_pack(h->p, h->stage, h->q[0] + (size_t)f * h->p.fsize, h->stream));
-        HIPCHK(h, launch_pack(h->p, h->stage, h->q[1] + (size_t)f * h->p.fsize, h->stream));
+        if (!h->mixed) {
+            HIPCHK(h, launch_pack(h->p, h->stage, h->q[0] + off, h->stream));
+            HIPCHK(h, launch_pack(h->p, h->stage, h->q[1] + off, h->stream));
+        } else {
+            HIPCHK(h, launch_pack_mixed(h->p, h->stage, bgcol(h, f), h->hq[1] + off, h->dr + off, h->stream));
+            HIPCHK(h, launch_pack_mixed(h->p, h->stage, bgcol(h, f), h->hq[0] + off, h->dr + off, h->stream));
+        }
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->cur = 0;
@@ -193,33 +288,55 @@ extern "C" int mhd_download(mhd_t *h, double *ru, double *rv, double *rw, double
     HIPCHK(h, hipSetDevice(h->dev));
     for (int f = 0; f < nf; ++f) {
         if (!dst[f]) continue;
-        HIPCHK(h, launch_unpack(h->p, h->q[h->cur] + (size_t)f * h->p.fsize, h->stage, 1, h->stream));
+        const size_t off = (size_t)f * h->p.fsize;
+        if (!h->mixed) HIPCHK(h, launch_unpack(h->p, h->q[h->cur] + off, h->stage, 1, h->stream));
+        else HIPCHK(h, launch_unpack_mixed(h->p, h->hq[h->cur] + off, h->dr + off, bgcol(h, f), h->stage, h->stream));
         HIPCHK(h, hipMemcpyAsync(dst[f], h->stage, pbytes, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
     }
     return MHD_OK;
 }
 
-// One STEP.  The CFL limits of the state a step starts from are reduced by
-// the previous step's stage-3 and boundary kernels; only the first step
-// after an upload runs the stand-alone reduction.
-static int enqueue_step(mhd_t *h) {
+// The launches of one STEP, optionally bracketed by events (ev[0..13]:
+// CFL, then stage/BCON pairs of the three stages).  The CFL limits of the
+// state a step starts from are reduced by the previous step's stage-3 and
+// boundary kernels; only the first step after an upload runs the
+// stand-alone reduction.
+static hipError_t launches(mhd_t *h, hipEvent_t *ev) {
     const Params &p = h->p;
-    double *qa = h->q[h->cur], *qb = h->q[1 - h->cur];
+    hipError_t e = hipSuccess;
+    auto rec = [&](int i) {
+        if (ev && e == hipSuccess) e = hipEventRecord(ev[i], h->stream);
+    };
+    rec(0);
     if (!h->cfl_valid) {
-        HIPCHK(h, launch_cfl(p, h->m, qa, h->sc, h->stream));
-        HIPCHK(h, launch_cfl_final(p, h->sc, false, h->stream));
+        if (!h->mixed) e = launch_cfl(p, h->m, h->q[h->cur], h->sc, h->stream);
+        else e = launch_cfl_mixed(p, h->m, h->bg, h->hq[h->cur], h->dr, h->sc, h->stream);
+        if (e == hipSuccess) e = launch_cfl_final(p, h->sc, false, h->stream);
     }
-    HIPCHK(h, launch_stage(1, p, h->m, qa, qb, h->z, nullptr, h->sc, h->stream));
-    HIPCHK(h, launch_bcon(p, h->m, qa, qb, h->sc, false, h->stream));
-    HIPCHK(h, launch_stage(2, p, h->m, qb, qa, h->z, nullptr, h->sc, h->stream));
-    HIPCHK(h, launch_bcon(p, h->m, qb, qa, h->sc, false, h->stream));
-    HIPCHK(h, launch_stage(3, p, h->m, qa, qb, h->z, nullptr, h->sc, h->stream));
-    HIPCHK(h, launch_bcon(p, h->m, qa, qb, h->sc, true, h->stream));
-    HIPCHK(h, launch_cfl_final(p, h->sc, true, h->stream));
-    h->cur = 1 - h->cur;
-    h->cfl_valid = true;
-    return MHD_OK;
+    rec(1);
+    int a = h->cur, b = 1 - h->cur;
+    for (int st = 1; st <= 3 && e == hipSuccess; ++st) {
+        rec(2 + 4 * (st - 1));
+        if (!h->mixed) e = launch_stage(st, p, h->m, h->q[a], h->q[b], h->z, nullptr, h->sc, h->stream);
+        else e = launch_stage_mixed(st, p, h->m, h->bg, h->hq[a], h->hq[b], h->dr, h->sc, h->stream);
+        rec(3 + 4 * (st - 1));
+        rec(4 + 4 * (st - 1));
+        if (e == hipSuccess) {
+            if (!h->mixed) e = launch_bcon(p, h->m, h->q[a], h->q[b], h->sc, st == 3, h->stream);
+            else e = launch_bcon_mixed(p, h->m, h->bg, h->hq[b], h->dr, h->sc, st == 3, h->stream);
+        }
+        rec(5 + 4 * (st - 1));
+        const int t = a;
+        a = b;
+        b = t;
+    }
+    if (e == hipSuccess) e = launch_cfl_final(p, h->sc, true, h->stream);
+    if (e == hipSuccess) {
+        h->cur = 1 - h->cur;
+        h->cfl_valid = true;
+    }
+    return e;
 }
 
 static int finish(mhd_t *h, double *dt, double *umach) {
@@ -229,7 +346,8 @@ static int finish(mhd_t *h, double *dt, double *umach) {
     if (dt) *dt = s.dt_used;
     if (umach) *umach = s.umach_used;
     if (s.nonfinite)
-        return fail(h, MHD_ERR_NONFINITE, "non-finite CFL limit or Mach number (DT=%g, UMACH=%g)", s.dt, s.umach);
+        return fail(h, MHD_ERR_NONFINITE, "non-finite CFL limit or Mach number (DT=%g, UMACH=%g)", s.dt_used,
+                    s.umach_used);
     return MHD_OK;
 }
 
@@ -244,7 +362,7 @@ extern "C" int mhd_step(mhd_t *h, double *dt, double *umach) {
     int rc = ready(h, "mhd_step");
     if (rc) return rc;
     HIPCHK(h, hipSetDevice(h->dev));
-    if ((rc = enqueue_step(h))) return rc;
+    HIPCHK(h, launches(h, nullptr));
     return finish(h, dt, umach);
 }
 
@@ -253,8 +371,7 @@ extern "C" int mhd_steps(mhd_t *h, int n, double *dt, double *umach) {
     if (rc) return rc;
     if (n < 0) return fail(h, MHD_ERR_ARG, "mhd_steps: n=%d", n);
     HIPCHK(h, hipSetDevice(h->dev));
-    for (int i = 0; i < n; ++i)
-        if ((rc = enqueue_step(h))) return rc;
+    for (int i = 0; i < n; ++i) HIPCHK(h, launches(h, nullptr));
     return finish(h, dt, umach);
 }
 
@@ -264,63 +381,38 @@ extern "C" int mhd_time_steps(mhd_t *h, int n, double *ms_step, double *ms_stage
     if (rc) return rc;
     if (n < 1) return fail(h, MHD_ERR_ARG, "mhd_time_steps: n=%d", n);
     HIPCHK(h, hipSetDevice(h->dev));
-    // event pairs: per step, CFL, 3 stages, 3 BCON launches
-    const int per = 7;
-    hipEvent_t *ev = new (std::nothrow) hipEvent_t[(size_t)2 * per * n + 2];
-    if (!ev) return fail(h, MHD_ERR_ARG, "mhd_time_steps: out of host memory");
-    const int nev = 2 * per * n + 2;
-    int made = 0;
+    constexpr int PER = 14;
+    std::vector<hipEvent_t> ev((size_t)PER * n + 2);
+    size_t made = 0;
     hipError_t e = hipSuccess;
-    for (; made < nev && e == hipSuccess; ++made) e = hipEventCreate(&ev[made]);
+    for (; made < ev.size() && e == hipSuccess; ++made) e = hipEventCreate(&ev[made]);
     if (e != hipSuccess) {
-        for (int i = 0; i < made - 1; ++i) hipEventDestroy(ev[i]);
-        delete[] ev;
+        for (size_t i = 0; i + 1 < made; ++i) (void)hipEventDestroy(ev[i]);
         return fail(h, MHD_ERR_HIP, "mhd_time_steps: hipEventCreate: %s", hipGetErrorString(e));
     }
-    const Params &p = h->p;
-    int ie = 2;
     e = hipEventRecord(ev[0], h->stream);
-    for (int s = 0; s < n && e == hipSuccess; ++s) {
-        double *qa = h->q[h->cur], *qb = h->q[1 - h->cur];
-        double *in[3] = {qa, qb, qa}, *out[3] = {qb, qa, qb};
-        (void)hipEventRecord(ev[ie++], h->stream);
-        if (!h->cfl_valid) {
-            e = launch_cfl(p, h->m, qa, h->sc, h->stream);
-            if (e == hipSuccess) e = launch_cfl_final(p, h->sc, false, h->stream);
-        }
-        (void)hipEventRecord(ev[ie++], h->stream);
-        for (int st = 0; st < 3 && e == hipSuccess; ++st) {
-            (void)hipEventRecord(ev[ie++], h->stream);
-            e = launch_stage(st + 1, p, h->m, in[st], out[st], h->z, nullptr, h->sc, h->stream);
-            (void)hipEventRecord(ev[ie++], h->stream);
-            if (e != hipSuccess) break;
-            (void)hipEventRecord(ev[ie++], h->stream);
-            e = launch_bcon(p, h->m, in[st], out[st], h->sc, st == 2, h->stream);
-            (void)hipEventRecord(ev[ie++], h->stream);
-        }
-        if (e == hipSuccess) e = launch_cfl_final(p, h->sc, true, h->stream);
-        h->cur = 1 - h->cur;
-        h->cfl_valid = true;
-    }
+    for (int s = 0; s < n && e == hipSuccess; ++s) e = launches(h, &ev[2 + (size_t)PER * s]);
     if (e == hipSuccess) e = hipEventRecord(ev[1], h->stream);
     if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
     double tstage = 0, tcfl = 0, tbcon = 0;
-    float ms = 0;
+    float ms = 0, total = 0;
     if (e == hipSuccess) {
-        int j = 2;
         for (int s = 0; s < n; ++s) {
-            hipEventElapsedTime(&ms, ev[j], ev[j + 1]); tcfl += ms; j += 2;
+            hipEvent_t *x = &ev[2 + (size_t)PER * s];
+            (void)hipEventElapsedTime(&ms, x[0], x[1]);
+            tcfl += ms;
             for (int st = 0; st < 3; ++st) {
-                hipEventElapsedTime(&ms, ev[j], ev[j + 1]); tstage += ms; j += 2;
-                hipEventElapsedTime(&ms, ev[j], ev[j + 1]); tbcon += ms; j += 2;
+                (void)hipEventElapsedTime(&ms, x[2 + 4 * st], x[3 + 4 * st]);
+                tstage += ms;
+                (void)hipEventElapsedTime(&ms, x[4 + 4 * st], x[5 + 4 * st]);
+                tbcon += ms;
             }
         }
-        hipEventElapsedTime(&ms, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&total, ev[0], ev[1]);
     }
-    for (int i = 0; i < nev; ++i) hipEventDestroy(ev[i]);
-    delete[] ev;
+    for (auto &x : ev) (void)hipEventDestroy(x);
     if (e != hipSuccess) return fail(h, MHD_ERR_HIP, "mhd_time_steps: %s", hipGetErrorString(e));
-    if (ms_step) *ms_step = ms / n;
+    if (ms_step) *ms_step = total / n;
     if (ms_stage) *ms_stage = tstage / (3.0 * n);
     if (ms_cfl) *ms_cfl = tcfl / n;
     if (ms_bcon) *ms_bcon = tbcon / (3.0 * n);
@@ -358,6 +450,7 @@ extern "C" int mhd_fluxes(mhd_t *h, double *fu, double *fv, double *fw, double *
                           double *ww1, double *ww2, double *ww3) {
     int rc = ready(h, "mhd_fluxes");
     if (rc) return rc;
+    if (h->mixed) return fail(h, MHD_ERR_SCOPE, "mhd_fluxes: only for MHD_FP64 handles");
     HIPCHK(h, hipSetDevice(h->dev));
     const int nf = nfields(h);
     if (!h->f) HIPCHK(h, hipMalloc(&h->f, (size_t)h->p.fsize * sizeof(double) * nf));

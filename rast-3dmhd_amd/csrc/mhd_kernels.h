@@ -52,6 +52,15 @@ struct DevScalars {
     int nonfinite;
 };
 
+// Hydrostatic background of the mixed form, per plane k = 0..npz+1:
+// FP64 rho0, T0 and the background's own discrete residuals of the momentum
+// z equation and the temperature equation; FP32 copies and per-plane
+// coefficients for the perturbation stencil.
+struct Background {
+    const double *ro, *tt, *fwbg, *ftbg;
+    const float *ro32, *tt32, *dt0dz32, *rinv0_32, *s0_32;
+};
+
 __host__ __device__ inline unsigned long long dkey(double v) {
     unsigned long long b = __builtin_bit_cast(unsigned long long, v);
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
@@ -70,5 +79,17 @@ hipError_t launch_cfl(const Params &p, const Metrics &m, const double *q, DevSca
 hipError_t launch_cfl_final(const Params &p, DevScalars *sc, bool clock, hipStream_t s);
 hipError_t launch_pack(const Params &p, const double *padded, double *field, hipStream_t s);
 hipError_t launch_unpack(const Params &p, const double *field, double *padded, int mode, hipStream_t s);
+
+// Mixed-precision form (mhd_mixed.hip).
+hipError_t launch_stage_mixed(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,
+                              float *hout, float *dr, DevScalars *sc, hipStream_t s);
+hipError_t launch_bcon_mixed(const Params &p, const Metrics &m, const Background &bg, float *h, float *dr,
+                             DevScalars *sc, bool last, hipStream_t s);
+hipError_t launch_cfl_mixed(const Params &p, const Metrics &m, const Background &bg, const float *h,
+                            const float *dr, DevScalars *sc, hipStream_t s);
+hipError_t launch_pack_mixed(const Params &p, const double *padded, const double *bgcol, float *h, float *dr,
+                             hipStream_t s);
+hipError_t launch_unpack_mixed(const Params &p, const float *h, const float *dr, const double *bgcol,
+                               double *padded, hipStream_t s);
 
 }  // namespace rmhd

@@ -1,0 +1,563 @@
+// Mixed-precision form of the step (no counterpart in the reference).
+//
+// The state is carried as FP32 perturbations h about an FP64 hydrostatic
+// background (rho0(z), T0(z); zero momentum and field), and the right-hand
+// side is evaluated in FP32 in perturbation form: F(bg + h) = F_bg(z) + F'(h),
+// where F_bg (nonzero only for the momentum z component and the temperature)
+// is the background's own discrete residual, formed once per plane in FP64.
+// The RK3 accumulation runs in FP64 in registers.  The second register D
+// (FP32) carries, per Wray's low-storage scheme, the zeta*dt*F increment of
+// the next stage plus the FP32 rounding residual of h, and after the last
+// stage the rounding residual alone, so that (h, D) is a compensated pair
+// and the storage rounding does not accumulate from step to step.
+// The CFL reduction, the boundary planes and the background are FP64.
+//
+// Traffic: every stage reads h (stencil) and D and writes h and D:
+// 4 x 8 fields x 4 B = 128 B/zone/stage, 384 B/zone/step (FP64: 640).
+
+#include "mhd_device.h"
+
+namespace rmhd {
+
+constexpr int MTX = 64;
+
+template <int TY_>
+struct MTile {
+    static constexpr int TY = TY_;
+    static constexpr int NT = MTX * TY;
+    static constexpr int SW = MTX + 2;
+    static constexpr int SH = TY + 2;
+    static constexpr int SP = SW * SH;
+    static constexpr int NHALO = SP - NT;
+    static_assert(NHALO <= NT, "one halo point per thread at most");
+};
+
+template <int TY, bool MAG>
+constexpr int mixed_lds_bytes() {
+    return (2 * (MAG ? 8 : 5) + 9) * MTile<TY>::SP * (int)sizeof(float);
+}
+
+struct RawF {
+    float v[NFIELD];
+};
+
+template <bool MAG>
+__device__ __forceinline__ void load_rawf(const float *__restrict__ q, long long fs, long long off, RawF &r) {
+#pragma unroll
+    for (int f = 0; f < (MAG ? 8 : 5); ++f) r.v[f] = q[f * fs + off];
+}
+
+// MODE 1..3: RK3 stage s of the mixed form.
+template <int TY, bool MAG, int MODE>
+__global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metrics m, const Background bg,
+                                                     const float *__restrict__ hin, float *__restrict__ hout,
+                                                     float *__restrict__ dr, DevScalars *__restrict__ sc,
+                                                     int kchunk) {
+    using T = MTile<TY>;
+    constexpr int SW = T::SW, SP = T::SP;
+    constexpr int NF = MAG ? 8 : 5;
+    extern __shared__ float ldsf[];
+    float *const s_cur = ldsf;
+    float *const s_ring = ldsf + 2 * NF * SP;
+
+    const int t = threadIdx.x;
+    const int npx = p.npx, npy = p.npy, npz = p.npz;
+    const long long fs = p.fsize, pl = p.plane;
+    const int ntx = (npx + MTX - 1) / MTX, nty = (npy + TY - 1) / TY;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int tz = bid / (ntx * nty);
+    const int rem = bid - tz * (ntx * nty);
+    const int tyb = rem / ntx, txb = rem - tyb * ntx;
+    const int i0 = txb * MTX, j0 = tyb * TY;
+    const int kbeg = 1 + tz * kchunk;
+    const int kend = min(npz, kbeg + kchunk - 1);
+    if (kbeg > npz) return;
+
+    const int olx = t % MTX + 1, oly = t / MTX + 1;
+    const int opt = oly * SW + olx;
+    const int gi = i0 + olx - 1, gj = j0 + oly - 1;
+    const bool valid = (gi < npx) && (gj < npy);
+    const long long ooff = (long long)wrapi(gj, npy) * npx + wrapi(gi, npx);
+    int hpt = -1;
+    long long hoff = 0;
+    if (t < T::NHALO) {
+        int hlx, hly;
+        if (t < SW) { hlx = t; hly = 0; }
+        else if (t < 2 * SW) { hlx = t - SW; hly = T::SH - 1; }
+        else if (t < 2 * SW + TY) { hlx = 0; hly = t - 2 * SW + 1; }
+        else { hlx = SW - 1; hly = t - 2 * SW - TY + 1; }
+        hpt = hly * SW + hlx;
+        hoff = (long long)wrapi(j0 + hly - 1, npy) * npx + wrapi(i0 + hlx - 1, npx);
+    }
+    const bool has_h = hpt >= 0;
+
+    auto cur = [&](int k, int f, int pt) -> float & { return s_cur[((k & 1) * NF + f) * SP + pt]; };
+    auto ring = [&](int k, int c, int pt) -> float & { return s_ring[((k % 3) * 3 + c) * SP + pt]; };
+    auto put_cur = [&](int k, int pt, const RawF &r) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) cur(k, f, pt) = r.v[f];
+    };
+    auto put_ring = [&](int k, int pt, const RawF &r) {
+        const float ri = 1.0f / (bg.ro32[k] + r.v[3]);
+        ring(k, 0, pt) = r.v[0] * ri;
+        ring(k, 1, pt) = r.v[1] * ri;
+        ring(k, 2, pt) = r.v[2] * ri;
+    };
+
+    RawF rm_, rp_o, rp_h;
+    load_rawf<MAG>(hin, fs, (long long)(kbeg - 1) * pl + ooff, rm_);
+    put_ring(kbeg - 1, opt, rm_);
+    if (has_h) {
+        load_rawf<MAG>(hin, fs, (long long)(kbeg - 1) * pl + hoff, rp_h);
+        put_ring(kbeg - 1, hpt, rp_h);
+    }
+#pragma unroll 1
+    for (int kk = kbeg; kk <= kbeg + 1; ++kk) {
+        load_rawf<MAG>(hin, fs, (long long)kk * pl + ooff, rp_o);
+        put_cur(kk, opt, rp_o);
+        put_ring(kk, opt, rp_o);
+        if (has_h) {
+            load_rawf<MAG>(hin, fs, (long long)kk * pl + hoff, rp_h);
+            put_cur(kk, hpt, rp_h);
+            put_ring(kk, hpt, rp_h);
+        }
+    }
+
+    const float HX = (float)p.hx, H2X = (float)p.h2x, HY = (float)p.hy, H2Y = (float)p.h2y;
+    const float HZ = (float)p.hz, H2Z = (float)p.h2z;
+    const float C43 = (float)p.c43, OCV = (float)p.ocv, ORE = (float)p.ore, ORM = (float)p.orm;
+    const float OBETA = (float)p.obeta, GRAV = (float)p.grav;
+    const int gix = wrapi(gi, npx) + 1, gjy = wrapi(gj, npy) + 1;
+    const float dx = (float)m.dxxdx[gix], d2x = (float)m.d2xxdx2[gix];
+    const float dy = (float)m.dyydy[gjy], d2y = (float)m.d2yydy2[gjy];
+    const float hxdx = HX * dx, hydy = HY * dy;
+    const float h2xdx2 = H2X * dx * dx, h2ydy2 = H2Y * dy * dy;
+    const float hxd2x = HX * d2x, hyd2y = HY * d2y;
+    const double dt = sc->dt;
+    const double coefg = (MODE == 1 ? p.gam1 : MODE == 2 ? p.gam2 : p.gam3) * dt;
+    const double coefz = (MODE == 1 ? p.zeta1 : p.zeta2) * dt;
+    double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
+    const double c_ddx = MODE == 3 ? m.ddx[gix] : 0.0, c_ddy = MODE == 3 ? m.ddy[gjy] : 0.0;
+
+#pragma unroll 1
+    for (int k = kbeg; k <= kend; ++k) {
+        const bool pre = k < kend;
+        const long long o = (long long)k * pl + ooff;
+        if (pre) {
+            load_rawf<MAG>(hin, fs, (long long)(k + 2) * pl + ooff, rp_o);
+            if (has_h) load_rawf<MAG>(hin, fs, (long long)(k + 2) * pl + hoff, rp_h);
+        }
+        float dk_[8];
+        if (valid) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) dk_[f] = dr[f * fs + o];
+        }
+        __syncthreads();
+        if (valid) {
+            const bool top = (k == 1), bot = (k == npz);
+            const float dz = (float)m.dzzdz[k], d2z = (float)m.d2zzdz2[k];
+            const float rk = (float)m.rkapa[k], dkap = (float)m.dkapa[k];
+            const float hzdz = HZ * dz, hzd2z = HZ * d2z, h2zdz2 = H2Z * dz * dz;
+            const float r0 = bg.ro32[k], t0 = bg.tt32[k];
+            const float r0m = bg.ro32[k - 1], t0m = bg.tt32[k - 1];
+            const float r0p = bg.ro32[k + 1], t0p = bg.tt32[k + 1];
+#define CUR(f, d) cur(k, f, opt + (d))
+#define NXT(f) cur(k + 1, f, opt)
+#define RNG(kp, c, d) ring(kp, c, opt + (d))
+            float q0[8];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) q0[f] = CUR(f, 0);
+            const float rh0 = q0[3], th0 = q0[4];
+            const float tf0 = t0 + th0;                    // full temperature
+            const float rinv = 1.0f / (r0 + rh0);          // 1/rho
+            const float u0 = RNG(k, 0, 0), v0 = RNG(k, 1, 0), w0 = RNG(k, 2, 0);
+            const float um = RNG(k - 1, 0, 0), vm = RNG(k - 1, 1, 0), wm = RNG(k - 1, 2, 0);
+            const float up = RNG(k + 1, 0, 0), vp = RNG(k + 1, 1, 0), wp = RNG(k + 1, 2, 0);
+            const float uxp = RNG(k, 0, 1), uxm = RNG(k, 0, -1), uyp = RNG(k, 0, SW), uym = RNG(k, 0, -SW);
+            const float vxp = RNG(k, 1, 1), vxm = RNG(k, 1, -1), vyp = RNG(k, 1, SW), vym = RNG(k, 1, -SW);
+            const float wxp = RNG(k, 2, 1), wxm = RNG(k, 2, -1), wyp = RNG(k, 2, SW), wym = RNG(k, 2, -SW);
+            const float ru_m = rm_.v[0], rv_m = rm_.v[1], rw_m = rm_.v[2], rh_m = rm_.v[3], th_m = rm_.v[4];
+            const float ru_p = NXT(0), rv_p = NXT(1), rw_p = NXT(2), rh_p = NXT(3), th_p = NXT(4);
+            // pressure perturbation P' = rho0 T' + rho' (T0 + T')
+            auto pprime = [](float r0_, float t0_, float rh, float th) { return r0_ * th + rh * (t0_ + th); };
+            const float pm = pprime(r0m, t0m, rh_m, th_m), pp = pprime(r0p, t0p, rh_p, th_p);
+            const float pxm = pprime(r0, t0, CUR(3, -1), CUR(4, -1)), pxp = pprime(r0, t0, CUR(3, 1), CUR(4, 1));
+            const float pym = pprime(r0, t0, CUR(3, -SW), CUR(4, -SW)), pyp = pprime(r0, t0, CUR(3, SW), CUR(4, SW));
+
+            float rwk2 = 0.f, wk2 = 0.f, bxk2 = 0.f, bxk3 = 0.f, byk2 = 0.f, byk3 = 0.f;
+            if (top || bot) {
+                const int ka = top ? k + 2 : k - 2;
+                const int kb = top ? k + 3 : k - 3;
+                const long long oa = (long long)ka * pl + ooff, ob = (long long)kb * pl + ooff;
+                rwk2 = hin[2 * fs + oa];
+                wk2 = rwk2 * (1.0f / (bg.ro32[ka] + hin[3 * fs + oa]));
+                if (MAG) {
+                    bxk2 = hin[5 * fs + oa];
+                    byk2 = hin[6 * fs + oa];
+                    bxk3 = hin[5 * fs + ob];
+                    byk3 = hin[6 * fs + ob];
+                }
+            }
+
+            // continuity
+            float fr = (CUR(0, -1) - CUR(0, 1)) * hxdx - (CUR(1, SW) - CUR(1, -SW)) * hydy;
+            if (top) fr -= (4.0f * rw_p - rwk2) * hzdz;
+            else if (bot) fr -= (rwk2 - 4.0f * rw_m) * hzdz;
+            else fr -= (rw_p - rw_m) * hzdz;
+            // buoyancy and pressure gradient of the perturbation
+            float fw = GRAV * rh0 - (pp - pm) * hzdz;
+            float fv = (pym - pyp) * hydy;
+            float fu = (pxm - pxp) * hxdx;
+            // momentum advection
+            fu -= (CUR(0, 1) * uxp - CUR(0, -1) * uxm) * hxdx + (CUR(0, SW) * vyp - CUR(0, -SW) * vym) * hydy
+                  + (ru_p * wp - ru_m * wm) * hzdz;
+            fv -= (CUR(1, 1) * uxp - CUR(1, -1) * uxm) * hxdx + (CUR(1, SW) * vyp - CUR(1, -SW) * vym) * hydy
+                  + (rv_p * wp - rv_m * wm) * hzdz;
+            fw -= (CUR(2, 1) * uxp - CUR(2, -1) * uxm) * hxdx + (CUR(2, SW) * vyp - CUR(2, -SW) * vym) * hydy
+                  + (rw_p * wp - rw_m * wm) * hzdz;
+            // temperature advection, the background gradient included
+            const float txp = CUR(4, 1), txm = CUR(4, -1), typ = CUR(4, SW), tym = CUR(4, -SW);
+            const float dtz = (th_p - th_m) * hzdz;
+            float ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + bg.dt0dz32[k]));
+            // conduction of T': (1/rho) c [kappa lap T' + kappa' dT'/dz]; plus the
+            // change of the background's conduction with 1/rho: -rho'/(rho rho0) S0
+            {
+                const float lap = (txp - txm) * hxd2x + (txp - 2.0f * th0 + txm) * h2xdx2
+                                  + (typ - tym) * hyd2y + (typ - 2.0f * th0 + tym) * h2ydy2
+                                  + (th_p - th_m) * hzd2z + (th_p - 2.0f * th0 + th_m) * h2zdz2;
+                ft += rinv * (float)p.t_cond * (rk * lap + dkap * dtz) - rh0 * rinv * bg.rinv0_32[k] * bg.s0_32[k];
+            }
+            // viscous Laplacians
+            fu += ORE * (C43 * ((uxp - uxm) * hxd2x + (uxp - 2.0f * u0 + uxm) * h2xdx2)
+                         + (uyp - uym) * hyd2y + (uyp - 2.0f * u0 + uym) * h2ydy2
+                         + (up - um) * hzd2z + (up - 2.0f * u0 + um) * h2zdz2);
+            fv += ORE * ((vxp - vxm) * hxd2x + (vxp - 2.0f * v0 + vxm) * h2xdx2
+                         + C43 * ((vyp - vym) * hyd2y + (vyp - 2.0f * v0 + vym) * h2ydy2)
+                         + (vp - vm) * hzd2z + (vp - 2.0f * v0 + vm) * h2zdz2);
+            fw += ORE * ((wxp - wxm) * hxd2x + (wxp - 2.0f * w0 + wxm) * h2xdx2
+                         + (wyp - wym) * hyd2y + (wyp - 2.0f * w0 + wym) * h2ydy2
+                         + C43 * ((wp - wm) * hzd2z + (wp - 2.0f * w0 + wm) * h2zdz2));
+            // heating and grad div u cross terms
+            const float tv = (float)p.t_visc, tv43 = (float)p.t_visc43, tcr = (float)p.t_cross;
+            const float dxu = (uxp - uxm) * hxdx, dxv = (vxp - vxm) * hxdx, dxw = (wxp - wxm) * hxdx;
+            const float dyu = (uyp - uym) * hydy, dyv = (vyp - vym) * hydy, dyw = (wyp - wym) * hydy;
+            const float dzu = (up - um) * hzdz, dzv = (vp - vm) * hzdz, dzw = (wp - wm) * hzdz;
+            ft += (C43 * dxu * dxu + dxv * dxv + dxw * dxw + dyu * dyu + 2.0f * dyu * dxv) * rinv * tv
+                  + (dyv * dyv + dzw * dzw - dxu * dzw - dxu * dyv - dyv * dzw) * rinv * tv43
+                  + (dzu * dzu + dzv * dzv + dyw * dyw + 2.0f * dzv * dyw + 2.0f * dzu * dxw) * rinv * tv
+                  - OCV * (dxu + dyv + dzw) * tf0;
+            fu += tcr * (((RNG(k, 1, SW + 1) - RNG(k, 1, SW - 1)) - (RNG(k, 1, -SW + 1) - RNG(k, 1, -SW - 1))) * hxdx * hydy
+                         + ((RNG(k + 1, 2, 1) - RNG(k - 1, 2, 1)) - (RNG(k + 1, 2, -1) - RNG(k - 1, 2, -1))) * hzdz * hxdx);
+            fv += tcr * (((RNG(k, 0, SW + 1) - RNG(k, 0, SW - 1)) - (RNG(k, 0, -SW + 1) - RNG(k, 0, -SW - 1))) * hxdx * hydy
+                         + ((RNG(k + 1, 2, SW) - RNG(k - 1, 2, SW)) - (RNG(k + 1, 2, -SW) - RNG(k - 1, 2, -SW))) * hzdz * hydy);
+            fw += tcr * (((RNG(k + 1, 0, 1) - RNG(k - 1, 0, 1)) - (RNG(k + 1, 0, -1) - RNG(k - 1, 0, -1))) * hzdz * hxdx
+                         + ((RNG(k + 1, 1, SW) - RNG(k - 1, 1, SW)) - (RNG(k + 1, 1, -SW) - RNG(k - 1, 1, -SW))) * hzdz * hydy);
+            // Coriolis
+            if (p.lrot) {
+                const float omx = (float)p.omx, omz = (float)p.omz;
+                fu += omz * q0[1];
+                fv += -omz * q0[0] + omx * q0[2];
+                fw -= omx * q0[1];
+            }
+            // MHD
+            float w1 = 0.f, w2 = 0.f, w3 = 0.f;
+            if (MAG) {
+                const float bx0 = q0[5], by0 = q0[6], bz0 = q0[7];
+                const float bxm = rm_.v[5], bym = rm_.v[6], bzm = rm_.v[7];
+                const float bxp = NXT(5), byp = NXT(6), bzp = NXT(7);
+                const float bxxp = CUR(5, 1), bxxm = CUR(5, -1), bxyp = CUR(5, SW), bxym = CUR(5, -SW);
+                const float byxp = CUR(6, 1), byxm = CUR(6, -1), byyp = CUR(6, SW), byym = CUR(6, -SW);
+                const float bzxp = CUR(7, 1), bzxm = CUR(7, -1), bzyp = CUR(7, SW), bzym = CUR(7, -SW);
+                float dzw1 = dzw;
+                if (top) dzw1 = (4.0f * wp - wk2) * hzdz;
+                else if (bot) dzw1 = (wk2 - 4.0f * wm) * hzdz;
+                const float dxbx = (bxxp - bxxm) * hxdx, dxby = (byxp - byxm) * hxdx, dxbz = (bzxp - bzxm) * hxdx;
+                const float dybx = (bxyp - bxym) * hydy, dyby = (byyp - byym) * hydy, dybz = (bzyp - bzym) * hydy;
+                const float dzbx = (bxp - bxm) * hzdz, dzby = (byp - bym) * hzdz, dzbz = (bzp - bzm) * hzdz;
+                // induction (B.grad) u - B div u - (u.grad) B
+                w1 = by0 * dyu - bx0 * dyv + bz0 * dzu - bx0 * dzw1 - u0 * dxbx - v0 * dybx - w0 * dzbx;
+                w2 = bx0 * dxv - by0 * dxu + bz0 * dzv - by0 * dzw1 - u0 * dxby - v0 * dyby - w0 * dzby;
+                w3 = bx0 * dxw - bz0 * dxu + by0 * dyw - bz0 * dyv - u0 * dxbz - v0 * dybz - w0 * dzbz;
+                // Lorentz force
+                fu += OBETA * (by0 * (dybx - dxby) - bz0 * dxbz + bz0 * dzbx);
+                fv += OBETA * (bx0 * (dxby - dybx) + bz0 * (dzby - dybz));
+                fw += OBETA * (bx0 * dxbz - bx0 * dzbx - by0 * (dzby - dybz));
+                // Ohmic heating |J|^2 / rho
+                ft += ((dybx - dxby) * (dybx - dxby) + dxbz * dxbz + dzbx * dzbx - 2.0f * dzbx * dxbz
+                       + (dzby - dybz) * (dzby - dybz)) * rinv * (float)p.t_ohm;
+                // resistive diffusion, one-sided in z on the boundary planes for BX, BY
+                float d2zbx, d2zby;
+                if (top) {
+                    d2zbx = (-3.0f * bx0 + 4.0f * bxp - bxk2) * hzd2z + (2.0f * bx0 - 5.0f * bxp + 4.0f * bxk2 - bxk3) * h2zdz2;
+                    d2zby = (-3.0f * by0 + 4.0f * byp - byk2) * hzd2z + (2.0f * by0 - 5.0f * byp + 4.0f * byk2 - byk3) * h2zdz2;
+                } else if (bot) {
+                    d2zbx = (3.0f * bx0 - 4.0f * bxm + bxk2) * hzd2z + (2.0f * bx0 - 5.0f * bxm + 4.0f * bxk2 - bxk3) * h2zdz2;
+                    d2zby = (3.0f * by0 - 4.0f * bym + byk2) * hzd2z + (2.0f * by0 - 5.0f * bym + 4.0f * byk2 - byk3) * h2zdz2;
+                } else {
+                    d2zbx = (bxp - bxm) * hzd2z + (bxp - 2.0f * bx0 + bxm) * h2zdz2;
+                    d2zby = (byp - bym) * hzd2z + (byp - 2.0f * by0 + bym) * h2zdz2;
+                }
+                const float d2zbz = (bzp - bzm) * hzd2z + (bzp - 2.0f * bz0 + bzm) * h2zdz2;
+                w1 += ORM * ((bxxp - bxxm) * hxd2x + (bxxp - 2.0f * bx0 + bxxm) * h2xdx2
+                             + (bxyp - bxym) * hyd2y + (bxyp - 2.0f * bx0 + bxym) * h2ydy2 + d2zbx);
+                w2 += ORM * ((byxp - byxm) * hxd2x + (byxp - 2.0f * by0 + byxm) * h2xdx2
+                             + (byyp - byym) * hyd2y + (byyp - 2.0f * by0 + byym) * h2ydy2 + d2zby);
+                w3 += ORM * ((bzxp - bzxm) * hxd2x + (bzxp - 2.0f * bz0 + bzxm) * h2xdx2
+                             + (bzyp - bzym) * hyd2y + (bzyp - 2.0f * bz0 + bzym) * h2ydy2 + d2zbz);
+            }
+#undef CUR
+#undef NXT
+#undef RNG
+            // FP64 accumulation: F = F_bg(z) + F', X = h + D + gamma dt F,
+            // store h' = fl32(X) and carry the residual (+ zeta dt F) in D
+            double Fd[8] = {fu, fv, fw, fr, ft, w1, w2, w3};
+            Fd[2] += bg.fwbg[k];
+            Fd[4] += bg.ftbg[k];
+            double xq[8];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const double X = ((double)q0[f] + (double)dk_[f]) + coefg * Fd[f];
+                const float hn = (float)X;
+                const double r = X - (double)hn;
+                xq[f] = X;
+                __builtin_nontemporal_store(hn, &hout[f * fs + o]);
+                const float dn = (MODE < 3) ? (float)(r + coefz * Fd[f]) : (float)r;
+                __builtin_nontemporal_store(dn, &dr[f * fs + o]);
+            }
+            if (MODE == 3 && !top && !bot) {
+                double full[8];
+#pragma unroll
+                for (int f = 0; f < NF; ++f) full[f] = xq[f];
+                full[3] += bg.ro[k];
+                full[4] += bg.tt[k];
+                cfl_point<MAG>(p, full, c_ddx, c_ddy, m.ddz[k], m.rkapa[k], c_mach, c_w1, c_w2, c_w3, c_w4);
+            }
+#pragma unroll
+            for (int f = 0; f < NF; ++f) rm_.v[f] = q0[f];
+        }
+        __syncthreads();
+        if (pre) {
+            put_cur(k + 2, opt, rp_o);
+            put_ring(k + 2, opt, rp_o);
+            if (has_h) {
+                put_cur(k + 2, hpt, rp_h);
+                put_ring(k + 2, hpt, rp_h);
+            }
+        }
+    }
+    if (MODE == 3) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
+}
+
+// BCON of the mixed form, in FP64 on the full state bg + h (+ D after the
+// last stage, where D holds only the rounding residual).  New boundary
+// values are split again into h and the residual D.
+template <bool MAG, bool LAST>
+__global__ __launch_bounds__(256) void k_bcon_m(const Params p, const Metrics m, const Background bg,
+                                                float *__restrict__ h, float *__restrict__ dr,
+                                                DevScalars *__restrict__ sc) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
+    if (idx < p.npx * p.npy) {
+        const long long fs = p.fsize, pl = p.plane;
+        const int nz = p.npz;
+        const double C13 = p.c13, C23 = p.c23, C43 = p.c43;
+        auto full = [&](int f, int k) -> double {
+            const long long o = f * fs + (long long)k * pl + idx;
+            double v = (double)h[o];
+            if (LAST) v += (double)dr[o];
+            if (f == 3) v += bg.ro[k];
+            if (f == 4) v += bg.tt[k];
+            return v;
+        };
+        auto put = [&](int f, int k, double v) {
+            if (f == 3) v -= bg.ro[k];
+            if (f == 4) v -= bg.tt[k];
+            const long long o = f * fs + (long long)k * pl + idx;
+            const float hn = (float)v;
+            h[o] = hn;
+            dr[o] = (float)(v - (double)hn);
+        };
+        for (int side = 0; side < 2; ++side) {
+            const int a = side ? nz : 1, b = side ? nz - 1 : 2, c = side ? nz - 2 : 3;
+            const double roa = full(3, a), rob = full(3, b), roc = full(3, c);
+            put(0, a, roa * (C43 * full(0, b) / rob - C13 * full(0, c) / roc));
+            put(1, a, roa * (C43 * full(1, b) / rob - C13 * full(1, c) / roc));
+            put(2, a, 0.0);
+            const int code = side ? p.izc : p.itc;
+            double tnew;
+            if (code == 0) {
+                tnew = side ? p.tb : 1.0;
+            } else {
+                const double dz = p.dzf / m.dzzdz[a];
+                tnew = side ? C43 * full(4, b) - C13 * full(4, c) + C23 * dz * p.theta / m.rkapa[a]
+                            : C43 * full(4, b) - C13 * full(4, c) - C23 * dz * p.theta;
+            }
+            put(4, a, tnew);
+            if (MAG && p.ibc == 0) put(7, a, 0.0);
+        }
+        if (LAST) {
+            const int j = idx / p.npx, i = idx - j * p.npx;
+            for (int side = 0; side < 2; ++side) {
+                const int k = side ? nz : 1;
+                double v[8];
+                for (int f = 0; f < (MAG ? 8 : 5); ++f) v[f] = full(f, k);
+                cfl_point<MAG>(p, v, m.ddx[i + 1], m.ddy[j + 1], m.ddz[k], m.rkapa[k], c_mach, c_w1, c_w2,
+                                 c_w3, c_w4);
+            }
+        }
+    }
+    if (LAST) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
+}
+
+// Stand-alone CFL of the mixed state (first step after an upload).
+template <bool MAG>
+__global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, const Background bg,
+                                               const float *__restrict__ h, const float *__restrict__ dr,
+                                               DevScalars *__restrict__ sc) {
+    const long long fs = p.fsize, pl = p.plane;
+    const long long n = (long long)p.npz * pl;
+    double mach = 0.0, w1 = INFINITY, w2 = INFINITY, w3 = INFINITY, w4 = INFINITY;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long o = pl + idx;
+        const int k = (int)(idx / pl) + 1;
+        const long long r2 = idx - (long long)(k - 1) * pl;
+        const int j = (int)(r2 / p.npx), i = (int)(r2 - (long long)j * p.npx);
+        double v[8];
+        for (int f = 0; f < (MAG ? 8 : 5); ++f) v[f] = (double)h[f * fs + o] + (double)dr[f * fs + o];
+        v[3] += bg.ro[k];
+        v[4] += bg.tt[k];
+        cfl_point<MAG>(p, v, m.ddx[i + 1], m.ddy[j + 1], m.ddz[k], m.rkapa[k], mach, w1, w2, w3, w4);
+    }
+    cfl_reduce(mach, w1, w2, w3, w4, p.lmag, sc);
+}
+
+// padded FP64 field -> (h, D) with D the FP32 rounding residual
+__global__ void k_pack_m(const Params p, const double *__restrict__ padded, const double *__restrict__ bgcol,
+                         float *__restrict__ h, float *__restrict__ dr) {
+    const long long n = p.fsize;
+    const int nxp = p.npx + 2, nyp = p.npy + 2;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long k = idx / p.plane;
+        const long long r = idx - k * p.plane;
+        const int j = (int)(r / p.npx), i = (int)(r - (long long)j * p.npx);
+        double v = padded[(k * nyp + (j + 1)) * nxp + (i + 1)];
+        if (bgcol) v -= bgcol[k];
+        const float hn = (float)v;
+        h[idx] = hn;
+        dr[idx] = (float)(v - (double)hn);
+    }
+}
+
+// (h, D) -> padded FP64 field with periodic ghosts
+__global__ void k_unpack_m(const Params p, const float *__restrict__ h, const float *__restrict__ dr,
+                           const double *__restrict__ bgcol, double *__restrict__ padded) {
+    const int nxp = p.npx + 2, nyp = p.npy + 2, nzp = p.npz + 2;
+    const long long n = (long long)nxp * nyp * nzp;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const long long k = idx / ((long long)nxp * nyp);
+        const long long r = idx - k * nxp * nyp;
+        const int J = (int)(r / nxp), I = (int)(r - (long long)J * nxp);
+        const long long o = k * p.plane + (long long)wrapi(J - 1, p.npy) * p.npx + wrapi(I - 1, p.npx);
+        double v = (double)h[o] + (double)dr[o];
+        if (bgcol) v += bgcol[k];
+        padded[idx] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+static int chunk_m(const Params &p, int ty, int *nchunks) {
+    const int ntiles = ((p.npx + MTX - 1) / MTX) * ((p.npy + ty - 1) / ty);
+    int nc = (4096 + ntiles - 1) / ntiles;
+    int maxc = p.npz / 8;
+    if (maxc < 1) maxc = 1;
+    if (nc > maxc) nc = maxc;
+    if (nc < 1) nc = 1;
+    const int kc = (p.npz + nc - 1) / nc;
+    *nchunks = (p.npz + kc - 1) / kc;
+    return kc;
+}
+
+template <int TY, bool MAG, int MODE>
+static hipError_t launch_m(const Params &p, const Metrics &m, const Background &bg, const float *hin, float *hout,
+                           float *dr, DevScalars *sc, hipStream_t s) {
+    constexpr int lds = mixed_lds_bytes<TY, MAG>();
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_m<TY, MAG, MODE>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    int nch;
+    const int kc = chunk_m(p, TY, &nch);
+    const int ntiles = ((p.npx + MTX - 1) / MTX) * ((p.npy + TY - 1) / TY);
+    hipLaunchKernelGGL((k_stage_m<TY, MAG, MODE>), dim3(ntiles * nch), dim3(MTX * TY), lds, s, p, m, bg, hin, hout,
+                       dr, sc, kc);
+    return hipGetLastError();
+}
+
+template <int TY>
+static hipError_t launch_m_ty(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,
+                              float *hout, float *dr, DevScalars *sc, hipStream_t s) {
+    if (p.lmag) {
+        if (mode == 1) return launch_m<TY, true, 1>(p, m, bg, hin, hout, dr, sc, s);
+        if (mode == 2) return launch_m<TY, true, 2>(p, m, bg, hin, hout, dr, sc, s);
+        return launch_m<TY, true, 3>(p, m, bg, hin, hout, dr, sc, s);
+    }
+    if (mode == 1) return launch_m<TY, false, 1>(p, m, bg, hin, hout, dr, sc, s);
+    if (mode == 2) return launch_m<TY, false, 2>(p, m, bg, hin, hout, dr, sc, s);
+    return launch_m<TY, false, 3>(p, m, bg, hin, hout, dr, sc, s);
+}
+
+hipError_t launch_stage_mixed(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,
+                              float *hout, float *dr, DevScalars *sc, hipStream_t s) {
+    if (p.tile_y == 8) return launch_m_ty<8>(mode, p, m, bg, hin, hout, dr, sc, s);
+    return launch_m_ty<4>(mode, p, m, bg, hin, hout, dr, sc, s);
+}
+
+hipError_t launch_bcon_mixed(const Params &p, const Metrics &m, const Background &bg, float *h, float *dr,
+                             DevScalars *sc, bool last, hipStream_t s) {
+    const int n = p.npx * p.npy;
+    dim3 grid((n + 255) / 256), block(256);
+    if (p.lmag) {
+        if (last) hipLaunchKernelGGL((k_bcon_m<true, true>), grid, block, 0, s, p, m, bg, h, dr, sc);
+        else hipLaunchKernelGGL((k_bcon_m<true, false>), grid, block, 0, s, p, m, bg, h, dr, sc);
+    } else {
+        if (last) hipLaunchKernelGGL((k_bcon_m<false, true>), grid, block, 0, s, p, m, bg, h, dr, sc);
+        else hipLaunchKernelGGL((k_bcon_m<false, false>), grid, block, 0, s, p, m, bg, h, dr, sc);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_mixed(const Params &p, const Metrics &m, const Background &bg, const float *h,
+                            const float *dr, DevScalars *sc, hipStream_t s) {
+    const long long n = (long long)p.npz * p.plane;
+    long long nb = (n + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    if (p.lmag) hipLaunchKernelGGL(k_cfl_m<true>, dim3((unsigned)nb), dim3(256), 0, s, p, m, bg, h, dr, sc);
+    else hipLaunchKernelGGL(k_cfl_m<false>, dim3((unsigned)nb), dim3(256), 0, s, p, m, bg, h, dr, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_mixed(const Params &p, const double *padded, const double *bgcol, float *h, float *dr,
+                             hipStream_t s) {
+    long long nb = (p.fsize + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_pack_m, dim3((unsigned)nb), dim3(256), 0, s, p, padded, bgcol, h, dr);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_mixed(const Params &p, const float *h, const float *dr, const double *bgcol,
+                               double *padded, hipStream_t s) {
+    const long long n = (long long)(p.npx + 2) * (p.npy + 2) * (p.npz + 2);
+    long long nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_unpack_m, dim3((unsigned)nb), dim3(256), 0, s, p, h, dr, bgcol, padded);
+    return hipGetLastError();
+}
+
+}  // namespace rmhd

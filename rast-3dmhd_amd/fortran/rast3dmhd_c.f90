@@ -9,7 +9,7 @@ module rast3dmhd_c
     integer(c_int) :: npx, npy, npz
     integer(c_int) :: lmag, lrot
     integer(c_int) :: izcon, itcon, ibcon
-    integer(c_int) :: reserved
+    integer(c_int) :: precision
     real(c_double) :: hx, h2x, hy, h2y, hz, h2z, c13, c23, c43
     real(c_double) :: gam1, gam2, gam3, zeta1, zeta2
     real(c_double) :: cv, ocv, ore, re, repr, theta, grav, sf, gamma
@@ -34,6 +34,12 @@ module rast3dmhd_c
       type(c_ptr), value :: h
       real(c_double), intent(in) :: dxxdx(*), d2xxdx2(*), ddx(*), dyydy(*), d2yydy2(*), ddy(*)
       real(c_double), intent(in) :: dzzdz(*), d2zzdz2(*), ddz(*), rkapa(*), dkapa(*)
+    end function
+
+    integer(c_int) function mhd_set_background(h, ro0, tt0) bind(c, name='mhd_set_background')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: ro0(*), tt0(*)
     end function
 
     integer(c_int) function mhd_upload(h, ru, rv, rw, ro, tt, bx, by, bz) bind(c, name='mhd_upload')

@@ -91,7 +91,7 @@ C
          D%IZCON=IZC
          D%ITCON=ITC
          D%IBCON=IBC
-         D%RESERVED=0
+         D%PRECISION=0
          D%HX=HX
          D%H2X=H2X
          D%HY=HY

@@ -1,0 +1,115 @@
+"""Synthetic convection workloads for the benchmark, the accuracy report and
+the tests.
+
+`convection_box` builds the driver-side set-up the reference performs for a
+rotating MHD polytrope (3dmhd.f:105-279 with 3dmhdset.f's parameters):
+float32-folded constants, an arctan-stretched z grid (NGRID=1, max/min
+spacing 3), the m=1 polytrope of STATIC, a seeded uniform random T
+perturbation, and a seed field from a random divergence-free vector
+potential.  The initial data are synthetic; no input files are read.
+"""
+
+from __future__ import annotations
+
+import importlib
+
+import numpy as np
+
+
+def _mods():
+    return (importlib.import_module("rast-3dmhd_amd.config"),
+            importlib.import_module("rast-3dmhd_amd.ics"))
+
+
+def convection_box(npx: int, npy: int, npz: int, *, seed: int = 20260, lmag: bool = True, lrot: bool = True,
+                   contrast: float = 30.0, ra: float = 1e6, pr: float = 1.0, pm: float = 2.0, ta: float = 1e6,
+                   tilt_deg: float = 45.0, ampt: float = 1e-4, kmax: int = 8, ampb: float = 1e-3,
+                   ngrid: int = 1):
+    """(meta, fields) of an npx x npy x npz box: density contrast `contrast`,
+    Rayleigh number `ra`, Prandtl `pr`, magnetic Prandtl `pm`, Taylor `ta`,
+    rotation axis tilted `tilt_deg` from the vertical (latitude f-plane),
+    seed field in modes |k| <= kmax at amplitude ampb; z grid NGRID=1
+    (arctan stretch) or NGRID=3 (tanh stretch), max/min spacing 3."""
+    cfg, ics = _mods()
+    ph = cfg.physics(contrast=contrast, ra=ra, pr=pr, pm=pm, ta=ta, tilt_deg=tilt_deg)
+    par = {k: cfg.f32(v) for k, v in ph.items()}
+    mt = cfg.metrics(npx, npy, npz, xmax=par[17], ymax=par[18], zmax=par[19], ngrid=ngrid)
+    t, r = cfg.polytrope(mt["zee"], par[3], par[4])
+    meta = cfg.constants(npx, npy, npz, par, lmag=lmag, lrot=lrot, izcon=1, itcon=0, tb=float(t[-1]))
+    meta.update(mt)
+    st = ics.static_state(npx, npy, npz, t, r, ampt=ampt, lmag=lmag, seed=seed, fast_noise=True)
+    if lmag:
+        st["bx"], st["by"], st["bz"] = ics.vector_potential_field(meta, kmax=kmax, amp=ampb, seed=seed)
+    for k in ("ru", "rv", "rw", "ro", "tt"):
+        ics.periodic_fill(st[k])
+    return meta, st
+
+
+def stratified_box(npx: int, npy: int, npz: int, *, seed: int = 20260):
+    """Density contrast 10, Ra=1e5, Pr=1, Pm=1, Taylor 1e4 on a
+    vertical-axis f-plane, aspect 4, tanh-stretched z, T perturbation 1e-4,
+    seed field in modes |k| <= 4 at amplitude 1e-5 (the accuracy workload)."""
+    return convection_box(npx, npy, npz, seed=seed, contrast=10.0, ra=1e5, pr=1.0, pm=1.0, ta=1e4,
+                          tilt_deg=0.0, ampt=1e-4, kmax=4, ampb=1e-5, ngrid=3)
+
+
+def divergence_b(meta: dict, st: dict) -> np.ndarray:
+    """Centred-difference div B on the interior planes 2..npz-1 (padded
+    arrays with periodic x/y ghosts), with the grid metrics of the step."""
+    bx, by, bz = st["bx"], st["by"], st["bz"]
+    dx = np.asarray(meta["dxxdx"])[1:-1, None, None]
+    dy = np.asarray(meta["dyydy"])[None, 1:-1, None]
+    dz = np.asarray(meta["dzzdz"])[None, None, 2:-2]
+    hx, hy, hz = float(meta["hx"]), float(meta["hy"]), float(meta["hz"])
+    return ((bx[2:, 1:-1, 2:-2] - bx[:-2, 1:-1, 2:-2]) * hx * dx
+            + (by[1:-1, 2:, 2:-2] - by[1:-1, :-2, 2:-2]) * hy * dy
+            + (bz[1:-1, 1:-1, 3:-1] - bz[1:-1, 1:-1, 1:-3]) * hz * dz)
+
+
+def _rel(a, b):
+    n = np.sqrt(np.mean(b * b))
+    return float(np.sqrt(np.mean((a - b) ** 2)) / n) if n > 0 else float(np.sqrt(np.mean((a - b) ** 2)))
+
+
+def _fluct(b):
+    return b - b.mean(axis=(0, 1), keepdims=True)
+
+
+def field_errors(meta: dict, test: dict, ref: dict) -> dict:
+    """Errors of state `test` against state `ref` on the interior zones.
+
+    rel_l2[f]   : ||test - ref|| / ||ref|| per stored field;
+    fluct_l2[f] : ||test - ref|| / ||ref - horizontal mean of ref|| for RO, TT;
+    velocity    : ||u_test - u_ref|| / ||u_ref|| over the three components;
+    entropy     : ||s_test - s_ref|| / ||s_ref - horizontal mean||, with
+                  s = ln(T) / (gamma - 1) - ln(rho);
+    divb_max    : max |div B| of both states (centred differences)."""
+    sl = (slice(1, -1),) * 3
+    lmag = bool(int(meta.get("lmag", 0)))
+    names = ["ru", "rv", "rw", "ro", "tt"] + (["bx", "by", "bz"] if lmag else [])
+    out = {"rel_l2": {}, "rel_max": {}, "fluct_l2": {}}
+    for k in names:
+        a, b = test[k][sl], ref[k][sl]
+        out["rel_l2"][k] = _rel(a, b)
+        m = float(np.abs(b).max())
+        out["rel_max"][k] = float(np.abs(a - b).max() / m) if m > 0 else float(np.abs(a - b).max())
+    for k in ("ro", "tt"):
+        a, b = test[k][sl], ref[k][sl]
+        out["fluct_l2"][k] = float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(_fluct(b) ** 2)))
+    ua = np.stack([test[k][sl] / test["ro"][sl] for k in ("ru", "rv", "rw")])
+    ub = np.stack([ref[k][sl] / ref["ro"][sl] for k in ("ru", "rv", "rw")])
+    out["velocity"] = _rel(ua, ub)
+    g = float(meta["gamma"])
+    sa = np.log(test["tt"][sl]) / (g - 1.0) - np.log(test["ro"][sl])
+    sb = np.log(ref["tt"][sl]) / (g - 1.0) - np.log(ref["ro"][sl])
+    out["entropy"] = float(np.sqrt(np.mean((sa - sb) ** 2)) / np.sqrt(np.mean(_fluct(sb) ** 2)))
+    if lmag:
+        bmag = [np.abs(test[k][sl]).max() for k in ("bx", "by", "bz")]
+        db = np.stack([test[k][sl] - ref[k][sl] for k in ("bx", "by", "bz")])
+        bb = np.stack([ref[k][sl] for k in ("bx", "by", "bz")])
+        out["b_field"] = _rel(db + bb, bb)
+        out["divb_max"] = {"test": float(np.abs(divergence_b(meta, test)).max()),
+                           "ref": float(np.abs(divergence_b(meta, ref)).max()),
+                           "b_max": float(max(bmag))}
+    out["max_rel_l2"] = max(out["rel_l2"].values())
+    return out

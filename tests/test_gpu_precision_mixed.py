@@ -1,0 +1,143 @@
+"""The mixed-precision form (FP32 perturbations about an FP64 background,
+FP64 accumulation) against the FP64 form and the oracle.
+
+The mixed form has no counterpart in the reference; its check is the
+error bound: after 1000 steps each stored field must agree with the FP64
+result (bit-identical to the reference) to 1e-6 relative L2, and the state
+must match the oracle to the same bound.  Deterministic behaviour
+(batching, periodic shifts) is checked bit for bit.
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+
+import helpers as H
+from test_gpu_parity import random_box
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6  # relative L2 error per field, mixed vs FP64
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def wl():
+    return importlib.import_module("rast-3dmhd_amd.workload")
+
+
+def run(meta, st, precision, n):
+    with H.pkg().Box(meta, precision=precision) as box:
+        box.upload(st)
+        dt, um = box.steps(n)
+        return box.download(), dt, um, box.clock()
+
+
+def test_mixed_upload_download_round_trip():
+    say("case mixed round trip")
+    meta, st = random_box(70, 37, 20, True, True, seed=11)
+    with H.pkg().Box(meta, precision="mixed") as box:
+        box.upload(st)
+        got = box.download()
+    for k in H.FIELDS:
+        a, b = got[k], st[k]
+        scale = np.abs(b).max()
+        assert np.abs(a - b).max() <= 1e-14 * scale, (k, np.abs(a - b).max(), scale)
+
+
+@pytest.mark.parametrize("lmag,lrot", [(True, True), (False, False)])
+def test_mixed_vs_fp64_1000_steps(lmag, lrot):
+    say("case mixed vs fp64 1000 steps", lmag, lrot)
+    meta, st = wl().convection_box(96, 64, 48, seed=5, lmag=lmag, lrot=lrot)
+    ref, dt64, um64, c64 = run(meta, st, "fp64", 1000)
+    mix, dtm, umm, cm = run(meta, st, "mixed", 1000)
+    assert c64[0] == cm[0] == 1000
+    assert abs(cm[1] - c64[1]) <= 1e-6 * c64[1], (cm[1], c64[1])
+    assert abs(dtm - dt64) <= 1e-6 * dt64
+    err = wl().field_errors(meta, mix, ref)
+    say("rel_l2", err["rel_l2"], "velocity", err["velocity"], "entropy", err["entropy"])
+    for k, e in err["rel_l2"].items():
+        assert e <= TOL, (k, e)
+    assert err["velocity"] <= TOL
+    if lmag:
+        d = err["divb_max"]
+        assert abs(d["test"] - d["ref"]) <= 1e-3 * d["ref"] + 1e-12 * d["b_max"], d
+
+
+def test_mixed_vs_oracle():
+    """Mixed form against the C restatement of the reference step."""
+    say("case mixed vs oracle")
+    O = H.oracle_mod()
+    meta, st = random_box(32, 24, 16, True, True, seed=21)
+    o = O.Oracle(meta, st)
+    nsteps = 100
+    for _ in range(nsteps):
+        o.step()
+    mix, dt, um, clk = run(meta, st, "mixed", nsteps)
+    assert abs(dt - o.meta["dt"]) <= 1e-6 * o.meta["dt"]
+    assert abs(um - o.meta["umach"]) <= 1e-5 * o.meta["umach"]
+    ref = {k: o.arrays[k] for k in H.FIELDS}
+    err = wl().field_errors(meta, mix, ref)
+    say("rel_l2", err["rel_l2"])
+    for k, e in err["rel_l2"].items():
+        assert e <= TOL, (k, e)
+
+
+def test_mixed_batched_equal_single():
+    say("case mixed batched steps")
+    meta, st = random_box(40, 24, 16, True, True, seed=3)
+    with H.pkg().Box(meta, precision="mixed") as a, H.pkg().Box(meta, precision="mixed") as b:
+        a.upload(st)
+        b.upload(st)
+        for _ in range(4):
+            a.step()
+        b.steps(4)
+        sa, sb = a.download(), b.download()
+        assert a.clock() == b.clock()
+    for k in H.FIELDS:
+        assert np.array_equal(sa[k], sb[k]), k
+
+
+def test_mixed_shift_equivariance():
+    """A periodically shifted state steps to the shifted result bit for bit."""
+    say("case mixed shift 192x128x64")
+    meta, st = wl().convection_box(192, 128, 64, seed=9)
+    si, sj = 37, 51
+
+    def shift(a):
+        out = np.zeros_like(a)
+        out[1:-1, 1:-1, :] = np.roll(np.roll(a[1:-1, 1:-1, :], si, axis=0), sj, axis=1)
+        H.ics().periodic_fill(out)
+        return out
+
+    r1, d1, _, _ = run(meta, st, "mixed", 3)
+    r2, d2, _, _ = run(meta, {k: shift(v) for k, v in st.items()}, "mixed", 3)
+    assert d1 == d2
+    for k in H.FIELDS:
+        assert np.array_equal(shift(r1[k]), r2[k]), k
+
+
+def test_mixed_entry_point_errors():
+    say("case mixed errors")
+    pkg = H.pkg()
+    meta, st = random_box(16, 8, 8, True, False, seed=1)
+    with pkg.Box(meta, precision="mixed") as box:
+        with pytest.raises(pkg.MhdError) as e:
+            box.step()
+        assert e.value.code == pkg.MHD_ERR_STATE
+        box.upload(st)
+        with pytest.raises(pkg.MhdError) as e:
+            box.fluxes()
+        assert e.value.code == pkg.MHD_ERR_SCOPE
+        bad = np.ones(10)
+        bad[3] = 0.0
+        with pytest.raises(pkg.MhdError) as e:
+            box.set_background(bad, np.ones(10))
+        assert e.value.code == pkg.MHD_ERR_ARG
+    with pkg.Box(meta) as box:
+        with pytest.raises(pkg.MhdError) as e:
+            box.set_background(np.ones(10), np.ones(10))
+        assert e.value.code == pkg.MHD_ERR_STATE
