@@ -9,6 +9,10 @@ potential in modes |k|<=8 at amplitude 1e-3, 500 RK3 steps with adaptive dt.
 hydrostatic background, FP64 accumulation); the plain FP64 form (the
 drop-in, bit-identical to the reference) is timed on the same box beside it.
 
+Ensemble: 64 independent boxes (sizes 32..96 x 32..96 x 32..64, random Ra,
+Ta, contrast, own seeds and dt), mixed precision, autotuned tiling, HIP-graph
+replay, against the same boxes stepped one at a time.
+
 Accuracy of the mixed form: a 256x256x128 box (contrast 10, Ra 1e5, Pr 1,
 Pm 1, Ta 1e4 vertical axis, tanh z grid, seed field |k|<=4 at 1e-5) is run
 1000 steps in both forms and the per-field errors are reported.
@@ -87,17 +91,46 @@ def roofline(precision, zones, kt, shape):
             "ms_stage": kt["stage"], "ms_cfl": kt["cfl"], "ms_bcon": kt["bcon"], "ms_step_events": kt["step"]}
 
 
-def accuracy(npx=256, npy=256, npz=128, steps=1000):
+def accuracy(npx=256, npy=256, npz=128, steps=1000, fixed=True):
     acc = importlib.import_module("rast-3dmhd_amd.accuracy")
-    r = acc.compare(npx, npy, npz, steps, log=log)
+    r = acc.compare(npx, npy, npz, steps, log=log, fixed=fixed)
     e = r["errors"]
-    return {"box": r["box"], "steps": steps, "workload": r["workload"],
+    return {"box": r["box"], "steps": steps, "workload": r["workload"], "dt": r["dt"], "fixed_dt": r["fixed_dt"],
             "rel_l2": e["rel_l2"], "max_rel_l2": e["max_rel_l2"], "velocity_rel_l2": e["velocity"],
             "entropy_rel_fluct_l2": e["entropy"], "rho_rel_fluct_l2": e["fluct_l2"]["ro"],
             "tt_rel_fluct_l2": e["fluct_l2"]["tt"], "b_rel_l2": e.get("b_field"),
             "divb_max_mixed": e["divb_max"]["test"], "divb_max_fp64": e["divb_max"]["ref"],
             "timc_rel_diff": r["timc_rel_diff"],
             "ms_step_fp64": r["fp64"]["ms_step_events"], "ms_step_mixed": r["mixed"]["ms_step_events"]}
+
+
+def ensemble(pkg, nbox=64, steps=100, precision="mixed"):
+    """The random 64-box ensemble: autotuned tiling, graph-replayed steps,
+    against the same boxes stepped one at a time through the single-box
+    library."""
+    specs = wl().ensemble_specs(nbox)
+    boxes = [wl().ensemble_box(s) for s in specs]
+    with pkg.Ensemble([m for m, _ in boxes], precision=precision) as ens:
+        for b, (_, s) in enumerate(boxes):
+            ens.upload(b, s)
+        ens.steps(2)
+        report = ens.autotune(nsteps=3)
+        ms = ens.time_steps(steps)
+        zones = ens.zones()
+        tiling = ens.tiling()
+    ms_single = 0.0
+    for m, s in boxes:
+        with pkg.Box(m, precision=precision) as box:
+            box.upload(s)
+            box.steps(2)
+            ms_single += box.time_steps(10)["step"]
+    return {"boxes": nbox, "zones": zones, "precision": precision, "steps": steps,
+            "ms_per_step": ms, "value": zones / (ms * 1e-3), "unit": "zone-updates/s",
+            "hbm_frac_step": zones / (ms * 1e-3) * BYTES_PER_ZONE_STEP[precision] / HBM_PEAK,
+            "tiling": {"tile_y": tiling[0], "kdepth": tiling[1], "grid": tiling[2]},
+            "autotune_candidates": len(report),
+            "autotune_ms_range": [min(r["ms_step"] for r in report), max(r["ms_step"] for r in report)],
+            "ms_per_step_boxes_one_at_a_time": ms_single, "speedup_vs_one_at_a_time": ms_single / ms}
 
 
 def cpu_baseline_reference(exe, ranks, n1=1, n2=3, timeout=900):
@@ -144,6 +177,7 @@ def main():
     ap.add_argument("--npz", type=int, default=256)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--accuracy-steps", type=int, default=1000, help="0 skips the mixed-vs-FP64 error run")
+    ap.add_argument("--ensemble-boxes", type=int, default=64, help="0 skips the batched ensemble run")
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--ranks", type=int, default=16)
     args = ap.parse_args()
@@ -164,10 +198,16 @@ def main():
     head = forms[0]
     zu = {p: zones * args.steps / r["wall"] for p, r in res.items()}
 
+    ens = None
+    if args.ensemble_boxes > 0:
+        log(f"ensemble: {args.ensemble_boxes} boxes")
+        ens = ensemble(pkg, nbox=args.ensemble_boxes)
+
     acc = None
     if args.accuracy_steps > 0 and args.precision == "both":
         log(f"accuracy: mixed vs fp64, 256x256x128, {args.accuracy_steps} steps")
         acc = accuracy(steps=args.accuracy_steps)
+        acc["adaptive_dt"] = accuracy(steps=args.accuracy_steps, fixed=False)
 
     cpu = None
     if args.cpu_baseline != "none":
@@ -219,6 +259,7 @@ def main():
         "forms": forms_out,
         "mixed_speedup_vs_fp64": (zu["mixed"] / zu["fp64"]) if len(res) == 2 else None,
         "mixed_accuracy": acc,
+        "ensemble": ens,
     }
     print(json.dumps(out))
 

@@ -112,6 +112,13 @@ int mhd_steps(mhd_t *h, int n, double *dt, double *umach);
 int mhd_get_clock(mhd_t *h, int *nit, double *timc, double *timt);
 int mhd_set_clock(mhd_t *h, int nit, double timc, double timi);
 
+/* No counterpart in the reference (which always steps with the CFL DT,
+ * 3dmhdsub.f:827-871): dt > 0 makes every following step use this DT (the
+ * CFL/Mach reduction still runs; UMACH and the non-finite check are
+ * unchanged); dt = 0 returns to the CFL DT.  For runs at a fixed step and
+ * for comparing forms at equal simulated times. */
+int mhd_set_fixed_dt(mhd_t *h, double dt);
+
 /* One evaluation of the right-hand side on the uploaded state, without
  * stepping: FU,FV,FW,FR,FT and (LMAG) WW1,WW2,WW3 = dB/dt, as FLUXES
  * leaves them (3dmhdsub.f:1178-2637).  Interior values only; ghost entries

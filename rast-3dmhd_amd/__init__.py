@@ -67,13 +67,38 @@ def load(path: str | None = None):
     lib.mhd_get_clock.argtypes = [C.c_void_p, C.POINTER(C.c_int), _P, _P]
     lib.mhd_set_clock.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
     lib.mhd_fluxes.argtypes = [C.c_void_p] + [_P] * 8
+    lib.mhd_set_fixed_dt.argtypes = [C.c_void_p, C.c_double]
     lib.mhd_time_steps.argtypes = [C.c_void_p, C.c_int, _P, _P, _P, _P]
     lib.mhd_last_error.argtypes = [C.c_void_p]
     lib.mhd_last_error.restype = C.c_char_p
     lib.mhd_destroy.argtypes = [C.c_void_p]
     lib.mhd_destroy.restype = None
+    V, I, D = C.c_void_p, C.c_int, C.c_double
+    lib.mhd_ens_create.argtypes = [C.POINTER(MhdDesc), I, C.POINTER(V)]
+    lib.mhd_ens_set_metrics.argtypes = [V, I] + [_P] * 11
+    lib.mhd_ens_set_background.argtypes = [V, I, _P, _P]
+    lib.mhd_ens_upload.argtypes = [V, I] + [_P] * 8
+    lib.mhd_ens_download.argtypes = [V, I] + [_P] * 8
+    lib.mhd_ens_set_clock.argtypes = [V, I, I, D, D]
+    lib.mhd_ens_get_clock.argtypes = [V, I, C.POINTER(I), _P, _P]
+    lib.mhd_ens_set_fixed_dt.argtypes = [V, I, D]
+    lib.mhd_ens_steps.argtypes = [V, I, I, _P, _P]
+    lib.mhd_ens_set_tiling.argtypes = [V, I, I, I]
+    lib.mhd_ens_get_tiling.argtypes = [V, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+    lib.mhd_ens_autotune.argtypes = [V, I, C.c_char_p, I]
+    lib.mhd_ens_time_steps.argtypes = [V, I, _P]
+    lib.mhd_ens_zones.argtypes = [V]
+    lib.mhd_ens_zones.restype = C.c_longlong
+    lib.mhd_ens_last_error.argtypes = [V]
+    lib.mhd_ens_last_error.restype = C.c_char_p
+    lib.mhd_ens_destroy.argtypes = [V]
+    lib.mhd_ens_destroy.restype = None
+    for fn in ("mhd_ens_create", "mhd_ens_set_metrics", "mhd_ens_set_background", "mhd_ens_upload",
+               "mhd_ens_download", "mhd_ens_set_clock", "mhd_ens_get_clock", "mhd_ens_set_fixed_dt", "mhd_ens_steps", "mhd_ens_set_tiling",
+               "mhd_ens_get_tiling", "mhd_ens_autotune", "mhd_ens_time_steps"):
+        getattr(lib, fn).restype = C.c_int
     for fn in ("mhd_create", "mhd_set_metrics", "mhd_set_background", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
-               "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps"):
+               "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps", "mhd_set_fixed_dt"):
         getattr(lib, fn).restype = C.c_int
     _lib = lib
     return lib
@@ -145,9 +170,7 @@ class Box:
 
     def upload(self, fields: dict):
         if self.precision == "mixed" and not self._have_bg:
-            ro = np.asarray(fields["ro"], dtype=np.float64)
-            tt = np.asarray(fields["tt"], dtype=np.float64)
-            self.set_background(ro[1:-1, 1:-1, :].mean(axis=(0, 1)), tt[1:-1, 1:-1, :].mean(axis=(0, 1)))
+            self.set_background(*_horizontal_means(fields))
         arrs = [self._padded(fields[k]) if (k in fields and (self.lmag or k not in ("bx", "by", "bz"))) else None
                 for k in FIELDS]
         self._check(self.lib.mhd_upload(self.h, *[_ptr(a) for a in arrs]))
@@ -183,6 +206,10 @@ class Box:
     def set_clock(self, nit: int, timc: float, timi: float):
         self._check(self.lib.mhd_set_clock(self.h, int(nit), float(timc), float(timi)))
 
+    def set_fixed_dt(self, dt: float):
+        """dt > 0: step with this DT from now on; 0: back to the CFL DT."""
+        self._check(self.lib.mhd_set_fixed_dt(self.h, float(dt)))
+
     def fluxes(self) -> dict:
         out = {k: np.zeros(self.shape, order="F") for k in RHS}
         ptrs = [_ptr(out[k]) if (self.lmag or k not in ("ww1", "ww2", "ww3")) else None for k in RHS]
@@ -192,6 +219,130 @@ class Box:
     def close(self):
         if getattr(self, "h", None):
             self.lib.mhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _horizontal_means(fields: dict):
+    ro = np.asarray(fields["ro"], dtype=np.float64)
+    tt = np.asarray(fields["tt"], dtype=np.float64)
+    return ro[1:-1, 1:-1, :].mean(axis=(0, 1)), tt[1:-1, 1:-1, :].mean(axis=(0, 1))
+
+
+class Ensemble:
+    """Independent boxes stepped together on the current GPU
+    (include/rast3dmhd_ensemble.h): one persistent launch per RK stage over
+    all boxes, replayed from a captured HIP graph.  metas: one dict of
+    COMMON scalars + metrics per box (as for Box); all boxes share lmag and
+    the precision.  For the mixed form the background of each box is the
+    horizontal mean of its first upload unless set_background was called."""
+
+    def __init__(self, metas: list, precision: str = "fp64"):
+        self.lib = load()
+        self.metas = metas
+        self.precision = precision
+        self.descs = (MhdDesc * len(metas))(*[make_desc(m, precision) for m in metas])
+        self.shapes = [(d.npx + 2, d.npy + 2, d.npz + 2) for d in self.descs]
+        self.lmag = bool(self.descs[0].lmag)
+        self.n = len(metas)
+        h = C.c_void_p()
+        rc = self.lib.mhd_ens_create(self.descs, self.n, C.byref(h))
+        if rc:
+            raise MhdError(rc, self.lib.mhd_ens_last_error(None).decode())
+        self.h = h
+        self._have_bg = [False] * self.n
+        for b, m in enumerate(metas):
+            arrs = [np.ascontiguousarray(m[k], dtype=np.float64) for k in METRICS]
+            self._check(self.lib.mhd_ens_set_metrics(self.h, b, *[_ptr(a) for a in arrs]))
+
+    def _check(self, rc):
+        if rc:
+            raise MhdError(rc, self.lib.mhd_ens_last_error(self.h).decode())
+
+    def set_background(self, b: int, ro0, tt0):
+        ro0 = np.ascontiguousarray(ro0, dtype=np.float64)
+        tt0 = np.ascontiguousarray(tt0, dtype=np.float64)
+        nz = self.shapes[b][2]
+        if ro0.shape != (nz,) or tt0.shape != (nz,):
+            raise ValueError(f"background profiles of box {b} must have {nz} planes")
+        self._check(self.lib.mhd_ens_set_background(self.h, b, _ptr(ro0), _ptr(tt0)))
+        self._have_bg[b] = True
+
+    def upload(self, b: int, fields: dict):
+        if self.precision == "mixed" and not self._have_bg[b]:
+            self.set_background(b, *_horizontal_means(fields))
+        arrs = []
+        for k in FIELDS:
+            if k in ("bx", "by", "bz") and not self.lmag:
+                arrs.append(None)
+                continue
+            a = np.asfortranarray(np.asarray(fields[k], dtype=np.float64))
+            if a.shape != self.shapes[b]:
+                raise ValueError(f"box {b} {k}: shape {a.shape}, expected {self.shapes[b]}")
+            arrs.append(a)
+        self._check(self.lib.mhd_ens_upload(self.h, b, *[_ptr(a) for a in arrs]))
+
+    def download(self, b: int) -> dict:
+        out = {k: np.zeros(self.shapes[b], order="F") for k in FIELDS}
+        ptrs = [_ptr(out[k]) if (self.lmag or k not in ("bx", "by", "bz")) else None for k in FIELDS]
+        self._check(self.lib.mhd_ens_download(self.h, b, *ptrs))
+        return out
+
+    def steps(self, n: int, graph: bool = True):
+        dt = np.zeros(self.n)
+        um = np.zeros(self.n)
+        self._check(self.lib.mhd_ens_steps(self.h, int(n), 1 if graph else 0, _ptr(dt), _ptr(um)))
+        return dt, um
+
+    def clock(self, b: int):
+        nit, timc, timt = C.c_int(), C.c_double(), C.c_double()
+        self._check(self.lib.mhd_ens_get_clock(self.h, b, C.byref(nit), C.byref(timc), C.byref(timt)))
+        return nit.value, timc.value, timt.value
+
+    def set_clock(self, b: int, nit: int, timc: float, timi: float):
+        self._check(self.lib.mhd_ens_set_clock(self.h, b, int(nit), float(timc), float(timi)))
+
+    def set_fixed_dt(self, b: int, dt: float):
+        self._check(self.lib.mhd_ens_set_fixed_dt(self.h, b, float(dt)))
+
+    def set_tiling(self, tile_y: int, kdepth: int, grid: int):
+        self._check(self.lib.mhd_ens_set_tiling(self.h, int(tile_y), int(kdepth), int(grid)))
+
+    def tiling(self):
+        v = [C.c_int() for _ in range(3)]
+        self._check(self.lib.mhd_ens_get_tiling(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune(self, nsteps: int = 5):
+        """Times the candidate tilings, keeps the fastest (state untouched);
+        returns the list of candidates with their ms per step."""
+        import json
+        buf = C.create_string_buffer(16384)
+        self._check(self.lib.mhd_ens_autotune(self.h, int(nsteps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    def time_steps(self, n: int) -> float:
+        ms = C.c_double()
+        self._check(self.lib.mhd_ens_time_steps(self.h, int(n), C.byref(ms)))
+        return ms.value
+
+    def zones(self) -> int:
+        return int(self.lib.mhd_ens_zones(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mhd_ens_destroy(self.h)
             self.h = None
 
     def __del__(self):

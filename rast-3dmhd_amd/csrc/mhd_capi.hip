@@ -18,7 +18,7 @@
 #include <vector>
 
 #include "../../include/rast3dmhd.h"
-#include "mhd_kernels.h"
+#include "mhd_host.h"
 
 using namespace rmhd;
 
@@ -82,43 +82,17 @@ static void free_all(mhd_t *h) {
 extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
     if (!d || !out) return fail(nullptr, MHD_ERR_ARG, "mhd_create: null argument");
     *out = nullptr;
-    if (d->npx < 3 || d->npy < 3 || d->npz < 4)
-        return fail(nullptr, MHD_ERR_ARG, "mhd_create: box %dx%dx%d too small (need npx,npy>=3, npz>=4)",
-                    d->npx, d->npy, d->npz);
-    if (d->izcon != 0 && d->izcon != 1)
-        return fail(nullptr, MHD_ERR_SCOPE, "mhd_create: IZCON=%d (BCON: Invalid IZCON)", d->izcon);
-    if (d->itcon != 0 && d->itcon != 1)
-        return fail(nullptr, MHD_ERR_SCOPE, "mhd_create: ITCON=%d not supported (0 or 1)", d->itcon);
-    if (d->tp != 0.0)
-        return fail(nullptr, MHD_ERR_SCOPE, "mhd_create: plume boundary (TP != 0) not supported");
-    if (d->precision != MHD_FP64 && d->precision != MHD_MIXED)
-        return fail(nullptr, MHD_ERR_ARG, "mhd_create: precision %d unknown", d->precision);
+    {
+        char msg[400];
+        const int rc = check_desc(d, msg, sizeof msg);
+        if (rc) return fail(nullptr, rc, "mhd_create: %s", msg);
+    }
     mhd_t *h = new (std::nothrow) mhd_ctx();
     if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_create: out of host memory");
     h->desc = *d;
     h->mixed = d->precision == MHD_MIXED;
-    Params &p = h->p;
-    p.npx = d->npx; p.npy = d->npy; p.npz = d->npz;
-    p.lmag = d->lmag ? 1 : 0; p.lrot = d->lrot ? 1 : 0;
-    p.izc = d->izcon; p.itc = d->itcon; p.ibc = d->ibcon;
-    p.tile_y = 4;
-    if (const char *ty = getenv("RAST3DMHD_TILE_Y")) p.tile_y = atoi(ty) == 8 ? 8 : 4;
-    p.plane = (long long)d->npx * d->npy;
-    p.fsize = p.plane * (d->npz + 2);
-    p.hx = d->hx; p.h2x = d->h2x; p.hy = d->hy; p.h2y = d->h2y; p.hz = d->hz; p.h2z = d->h2z;
-    p.c13 = d->c13; p.c23 = d->c23; p.c43 = d->c43;
-    p.gam1 = d->gam1; p.gam2 = d->gam2; p.gam3 = d->gam3; p.zeta1 = d->zeta1; p.zeta2 = d->zeta2;
-    p.cv = d->cv; p.ocv = d->ocv; p.ore = d->ore; p.re = d->re; p.repr = d->repr;
-    p.theta = d->theta; p.grav = d->grav; p.sf = d->sf; p.gamma = d->gamma;
-    p.omx = d->lrot ? d->omx : 0.0; p.omz = d->lrot ? d->omz : 0.0;
-    p.orm = d->orm; p.rm = d->rm; p.obeta = d->obeta; p.tb = d->tb; p.dzf = d->dzf;
-    // products formed once per FLUXES / STEP call in the reference
-    p.ogamma = 1.0 / d->gamma;             // sub:744
-    p.t_cond = d->ocv / d->repr;           // sub:1584
-    p.t_visc = d->ocv * d->ore;            // sub:1904
-    p.t_visc43 = d->c43 * d->ocv * d->ore; // sub:1962
-    p.t_cross = d->c13 * d->ore;           // sub:1930
-    p.t_ohm = d->obeta * d->orm * d->ocv;  // sub:2287
+    h->p = params_from_desc(d);
+    const Params &p = h->p;
     int rc;
     if (hipGetDevice(&h->dev) != hipSuccess) {
         rc = fail(nullptr, MHD_ERR_HIP, "mhd_create: no HIP device");
@@ -151,9 +125,7 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
         delete h;
         return rc;
     }
-    DevScalars s0{};
-    s0.key_mach = dkey(0.0);
-    for (int n = 0; n < 4; ++n) s0.key_min[n] = dkey(INFINITY);
+    const DevScalars s0 = initial_scalars();
     e = hipMemcpy(h->sc, &s0, sizeof(s0), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         rc = fail(nullptr, MHD_ERR_HIP, "mhd_create: %s", hipGetErrorString(e));
@@ -171,20 +143,16 @@ extern "C" int mhd_set_metrics(mhd_t *h, const double *dxxdx, const double *d2xx
                                const double *rkapa, const double *dkapa) {
     if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_set_metrics: null handle");
     const double *src[11] = {dxxdx, d2xxdx2, ddx, dyydy, d2yydy2, ddy, dzzdz, d2zzdz2, ddz, rkapa, dkapa};
-    const size_t len[11] = {(size_t)h->p.npx + 2, (size_t)h->p.npx + 2, (size_t)h->p.npx + 2,
-                            (size_t)h->p.npy + 2, (size_t)h->p.npy + 2, (size_t)h->p.npy + 2,
-                            (size_t)h->p.npz + 2, (size_t)h->p.npz + 2, (size_t)h->p.npz + 2,
-                            (size_t)h->p.npz + 2, (size_t)h->p.npz + 2};
-    const double **dst[11] = {&h->m.dxxdx, &h->m.d2xxdx2, &h->m.ddx, &h->m.dyydy, &h->m.d2yydy2, &h->m.ddy,
-                              &h->m.dzzdz, &h->m.d2zzdz2, &h->m.ddz, &h->m.rkapa, &h->m.dkapa};
+    size_t len[11];
+    const Metrics mt = metric_table(h->p, h->met, len);
     HIPCHK(h, hipSetDevice(h->dev));
     size_t off = 0;
     for (int n = 0; n < 11; ++n) {
         if (!src[n]) return fail(h, MHD_ERR_ARG, "mhd_set_metrics: array %d is null", n);
         HIPCHK(h, hipMemcpyAsync(h->met + off, src[n], len[n] * sizeof(double), hipMemcpyHostToDevice, h->stream));
-        *dst[n] = h->met + off;
         off += len[n];
     }
+    h->m = mt;
     const size_t nz = (size_t)h->p.npz + 2;
     h->dzzdz.assign(dzzdz, dzzdz + nz);
     h->d2zzdz2.assign(d2zzdz2, d2zzdz2 + nz);
@@ -200,54 +168,19 @@ extern "C" int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0
     if (!h->mixed) return fail(h, MHD_ERR_STATE, "mhd_set_background: only for MHD_MIXED handles");
     if (!h->have_metrics) return fail(h, MHD_ERR_STATE, "mhd_set_background: set the metrics first");
     if (!ro0 || !tt0) return fail(h, MHD_ERR_ARG, "mhd_set_background: null array");
-    const Params &p = h->p;
-    const int nz = p.npz + 2;
-    std::vector<double> d(4 * (size_t)nz, 0.0);
-    std::vector<float> f(5 * (size_t)nz, 0.0f);
-    double *ro = d.data(), *tt = ro + nz, *fw = tt + nz, *ft = fw + nz;
-    float *ro32 = f.data(), *tt32 = ro32 + nz, *dtz = tt32 + nz, *ri = dtz + nz, *s0 = ri + nz;
-    for (int k = 0; k < nz; ++k) {
-        if (!(ro0[k] > 0.0)) return fail(h, MHD_ERR_ARG, "mhd_set_background: rho0[%d] <= 0", k);
-        ro[k] = ro0[k];
-        tt[k] = tt0[k];
-        ro32[k] = (float)ro0[k];
-        tt32[k] = (float)tt0[k];
-        ri[k] = (float)(1.0 / ro0[k]);
-    }
-    // the background's discrete residuals and coefficients (planes 1..npz)
-    for (int k = 1; k <= p.npz; ++k) {
-        const double dz = h->dzzdz[k], d2z = h->d2zzdz2[k];
-        const double g0 = (tt[k + 1] - tt[k - 1]) * p.hz * dz;
-        const double l0 = (tt[k + 1] - tt[k - 1]) * p.hz * d2z + (tt[k + 1] - 2.0 * tt[k] + tt[k - 1]) * p.h2z * dz * dz;
-        const double sz = p.t_cond * (h->rkapa[k] * l0 + h->dkapa[k] * g0);
-        dtz[k] = (float)g0;
-        s0[k] = (float)sz;
-        ft[k] = sz / ro[k];
-        fw[k] = p.grav * ro[k] - (ro[k + 1] * tt[k + 1] - ro[k - 1] * tt[k - 1]) * p.hz * dz;
-    }
+    std::vector<double> d;
+    std::vector<float> f;
+    const int bad = background_tables(h->p, h->dzzdz.data(), h->d2zzdz2.data(), h->rkapa.data(), h->dkapa.data(),
+                                      ro0, tt0, d, f);
+    if (bad >= 0) return fail(h, MHD_ERR_ARG, "mhd_set_background: rho0[%d] <= 0", bad);
     HIPCHK(h, hipSetDevice(h->dev));
     HIPCHK(h, hipMemcpyAsync(h->bgd, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->bgf, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    h->bg.ro = h->bgd;
-    h->bg.tt = h->bgd + nz;
-    h->bg.fwbg = h->bgd + 2 * nz;
-    h->bg.ftbg = h->bgd + 3 * nz;
-    h->bg.ro32 = h->bgf;
-    h->bg.tt32 = h->bgf + nz;
-    h->bg.dt0dz32 = h->bgf + 2 * nz;
-    h->bg.rinv0_32 = h->bgf + 3 * nz;
-    h->bg.s0_32 = h->bgf + 4 * nz;
+    h->bg = background_view(h->p.npz, h->bgd, h->bgf);
     h->have_bg = true;
     h->have_state = false;
     return MHD_OK;
-}
-
-// background column of field f (RO and TT only), or null
-static const double *bgcol(const mhd_t *h, int f) {
-    if (f == F_RO) return h->bg.ro;
-    if (f == F_TT) return h->bg.tt;
-    return nullptr;
 }
 
 extern "C" int mhd_upload(mhd_t *h, const double *ru, const double *rv, const double *rw, const double *ro,
@@ -267,8 +200,8 @@ extern "C" int mhd_upload(mhd_t *h, const double *ru, const double *rv, const do
             HIPCHK(h, launch_pack(h->p, h->stage, h->q[0] + off, h->stream));
             HIPCHK(h, launch_pack(h->p, h->stage, h->q[1] + off, h->stream));
         } else {
-            HIPCHK(h, launch_pack_mixed(h->p, h->stage, bgcol(h, f), h->hq[1] + off, h->dr + off, h->stream));
-            HIPCHK(h, launch_pack_mixed(h->p, h->stage, bgcol(h, f), h->hq[0] + off, h->dr + off, h->stream));
+            HIPCHK(h, launch_pack_mixed(h->p, h->stage, f, h->bg, h->hq[1], h->dr, h->stream));
+            HIPCHK(h, launch_pack_mixed(h->p, h->stage, f, h->bg, h->hq[0], h->dr, h->stream));
         }
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -290,7 +223,7 @@ extern "C" int mhd_download(mhd_t *h, double *ru, double *rv, double *rw, double
         if (!dst[f]) continue;
         const size_t off = (size_t)f * h->p.fsize;
         if (!h->mixed) HIPCHK(h, launch_unpack(h->p, h->q[h->cur] + off, h->stage, 1, h->stream));
-        else HIPCHK(h, launch_unpack_mixed(h->p, h->hq[h->cur] + off, h->dr + off, bgcol(h, f), h->stage, h->stream));
+        else HIPCHK(h, launch_unpack_mixed(h->p, h->hq[h->cur], h->dr, f, h->bg, h->stage, h->stream));
         HIPCHK(h, hipMemcpyAsync(dst[f], h->stage, pbytes, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
     }
@@ -441,6 +374,21 @@ extern "C" int mhd_set_clock(mhd_t *h, int nit, double timc, double timi) {
     s.timc = timc;
     s.timi = timi;
     s.timt = timi + timc;
+    HIPCHK(h, hipMemcpyAsync(h->sc, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return MHD_OK;
+}
+
+extern "C" int mhd_set_fixed_dt(mhd_t *h, double dt) {
+    if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_set_fixed_dt: null handle");
+    if (!(dt >= 0.0) || dt == INFINITY) return fail(h, MHD_ERR_ARG, "mhd_set_fixed_dt: dt=%g", dt);
+    DevScalars s;
+    HIPCHK(h, hipSetDevice(h->dev));
+    HIPCHK(h, hipMemcpyAsync(&s, h->sc, sizeof(s), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    s.fixed_dt = dt;
+    if (dt > 0.0) s.dt = dt;
+    else h->cfl_valid = false;  // the next step reduces the CFL DT again
     HIPCHK(h, hipMemcpyAsync(h->sc, &s, sizeof(s), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     return MHD_OK;

@@ -48,6 +48,7 @@ struct DevScalars {
     double timc, timi, timt;
     unsigned long long key_mach;      // max of OGAMMA*UU/TT
     unsigned long long key_min[4];    // mins of the CFL limits
+    double fixed_dt;                  // > 0: step with this dt instead of the CFL dt
     int nit;
     int nonfinite;
 };
@@ -59,6 +60,29 @@ struct DevScalars {
 struct Background {
     const double *ro, *tt, *fwbg, *ftbg;
     const float *ro32, *tt32, *dt0dz32, *rinv0_32, *s0_32;
+};
+
+// One box of an ensemble as the batched kernels see it (device memory).
+// FP64 boxes use q/z, mixed boxes h/dr and bg.
+struct BoxDev {
+    Params p;
+    Metrics m;
+    Background bg;
+    double *q[2];
+    double *z;
+    float *h[2];
+    float *dr;
+    DevScalars *sc;
+};
+
+// Work item of a batched stage launch: one tile column chunk of one box.
+struct StageItem {
+    int box, i0, j0, kbeg, kend;
+};
+
+// Work item of a batched boundary-plane launch: 256 columns of one box.
+struct ColumnItem {
+    int box, col0;
 };
 
 __host__ __device__ inline unsigned long long dkey(double v) {
@@ -80,6 +104,15 @@ hipError_t launch_cfl_final(const Params &p, DevScalars *sc, bool clock, hipStre
 hipError_t launch_pack(const Params &p, const double *padded, double *field, hipStream_t s);
 hipError_t launch_unpack(const Params &p, const double *field, double *padded, int mode, hipStream_t s);
 
+// Ensemble launchers (all boxes share lmag and the tile height).  Register
+// a of every box is the stage input.  grid is the persistent grid size.
+hipError_t launch_stage_batch(int mode, int tile_y, int lmag, const BoxDev *boxes, const StageItem *items,
+                              int nitems, int grid, int a, hipStream_t s);
+hipError_t launch_bcon_batch(int lmag, const BoxDev *boxes, const ColumnItem *items, int nitems, int a, bool cfl,
+                             hipStream_t s);
+hipError_t launch_cfl_final_batch(const BoxDev *boxes, int nbox, bool clock, hipStream_t s);
+int stage_lds(int tile_y, int lmag, int mixed);
+
 // Mixed-precision form (mhd_mixed.hip).
 hipError_t launch_stage_mixed(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,
                               float *hout, float *dr, DevScalars *sc, hipStream_t s);
@@ -87,9 +120,17 @@ hipError_t launch_bcon_mixed(const Params &p, const Metrics &m, const Background
                              DevScalars *sc, bool last, hipStream_t s);
 hipError_t launch_cfl_mixed(const Params &p, const Metrics &m, const Background &bg, const float *h,
                             const float *dr, DevScalars *sc, hipStream_t s);
-hipError_t launch_pack_mixed(const Params &p, const double *padded, const double *bgcol, float *h, float *dr,
-                             hipStream_t s);
-hipError_t launch_unpack_mixed(const Params &p, const float *h, const float *dr, const double *bgcol,
+// Conversion of field `field` between a padded FP64 array and the stored
+// values / residuals of the registers hreg, dreg (base pointers, all
+// fields).  Pack rho (field 3) before T (field 4).
+hipError_t launch_pack_mixed(const Params &p, const double *padded, int field, const Background &bg, float *hreg,
+                             float *dreg, hipStream_t s);
+hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const float *dreg, int field, const Background &bg,
                                double *padded, hipStream_t s);
+
+hipError_t launch_stage_batch_mixed(int mode, int tile_y, int lmag, const BoxDev *boxes, const StageItem *items,
+                                    int nitems, int grid, int a, hipStream_t s);
+hipError_t launch_bcon_batch_mixed(int lmag, const BoxDev *boxes, const ColumnItem *items, int nitems, int a,
+                                   bool last, hipStream_t s);
 
 }  // namespace rmhd

@@ -67,34 +67,24 @@ constexpr int stage_lds_bytes() {
     return (2 * (MAG ? 8 : 5) + 9) * Tile<TY>::SP * (int)sizeof(double);
 }
 
-// MODE 0: write the right-hand side F to fout.  MODE 1..3: RK3 stage s.
+// One tile column chunk of a stage: interior columns i0..i0+TX-1, rows
+// j0..j0+TY-1 (clipped to the box), planes kbeg..kend.  MODE 0 writes the
+// right-hand side F to fout; MODE 1..3 performs RK3 stage s.  The whole
+// workgroup calls it with the same arguments.
 template <int TY, bool MAG, int MODE>
-__global__ __launch_bounds__(TX *TY) void k_stage(const Params p, const Metrics m,
-                                                  const double *__restrict__ qin,
-                                                  double *__restrict__ qout,
-                                                  double *__restrict__ zr,
-                                                  double *__restrict__ fout,
-                                                  DevScalars *__restrict__ sc,
-                                                  int kchunk) {
+__device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, const double *__restrict__ qin,
+                                           double *__restrict__ qout, double *__restrict__ zr,
+                                           double *__restrict__ fout, DevScalars *__restrict__ sc, int i0, int j0,
+                                           int kbeg, int kend, double *lds) {
     using T = Tile<TY>;
     constexpr int SW = T::SW, SP = T::SP;
     constexpr int NF = MAG ? 8 : 5;
-    extern __shared__ double lds[];
     double *const s_cur = lds;                  // [2][NF][SP]: raw planes k, k+1 (by parity)
     double *const s_ring = lds + 2 * NF * SP;   // [3][3][SP]: u, v, w of planes k-1, k, k+1
 
     const int t = threadIdx.x;
     const int npx = p.npx, npy = p.npy, npz = p.npz;
     const long long fs = p.fsize, pl = p.plane;
-    const int ntx = (npx + TX - 1) / TX, nty = (npy + TY - 1) / TY;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int tz = bid / (ntx * nty);
-    const int rem = bid - tz * (ntx * nty);
-    const int tyb = rem / ntx, txb = rem - tyb * ntx;
-    const int i0 = txb * TX, j0 = tyb * TY;
-    const int kbeg = 1 + tz * kchunk;
-    const int kend = min(npz, kbeg + kchunk - 1);
-    if (kbeg > npz) return;
 
     // own point
     const int olx = t % TX + 1, oly = t / TX + 1;
@@ -460,15 +450,53 @@ __global__ __launch_bounds__(TX *TY) void k_stage(const Params p, const Metrics 
     if (MODE == 3) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
 }
 
+// Single box: workgroup -> (tile, z chunk), XCD-aware.
+template <int TY, bool MAG, int MODE>
+__global__ __launch_bounds__(TX *TY) void k_stage(const Params p, const Metrics m,
+                                                  const double *__restrict__ qin,
+                                                  double *__restrict__ qout,
+                                                  double *__restrict__ zr,
+                                                  double *__restrict__ fout,
+                                                  DevScalars *__restrict__ sc,
+                                                  int kchunk) {
+    extern __shared__ double lds[];
+    const int ntx = (p.npx + TX - 1) / TX, nty = (p.npy + TY - 1) / TY;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int tz = bid / (ntx * nty);
+    const int rem = bid - tz * (ntx * nty);
+    const int tyb = rem / ntx, txb = rem - tyb * ntx;
+    const int kbeg = 1 + tz * kchunk;
+    const int kend = min(p.npz, kbeg + kchunk - 1);
+    if (kbeg > p.npz) return;
+    stage_tile<TY, MAG, MODE>(p, m, qin, qout, zr, fout, sc, txb * TX, tyb * TY, kbeg, kend, lds);
+}
+
+// Ensemble: a persistent grid walks the work-item table of all boxes with a
+// static stride (deterministic; every workgroup leaves after its last item).
+// Register a of every box is the input, 1-a the output.
+template <int TY, bool MAG, int MODE>
+__global__ __launch_bounds__(TX *TY) void k_stage_batch(const BoxDev *__restrict__ boxes,
+                                                        const StageItem *__restrict__ items, int nitems, int a) {
+    extern __shared__ double lds[];
+    const int slot = xcd_remap(blockIdx.x, gridDim.x);
+    for (int w = slot; w < nitems; w += gridDim.x) {
+        const StageItem it = items[w];
+        const BoxDev &B = boxes[it.box];
+        __syncthreads();  // the previous item's last LDS reads are done
+        stage_tile<TY, MAG, MODE>(B.p, B.m, B.q[a], B.q[1 - a], B.z, nullptr, B.sc, it.i0, it.j0, it.kbeg, it.kend,
+                                  lds);
+    }
+}
+
 // BCON on the two boundary planes of the freshly written register, and the
 // in-place 1/rho of the untouched z ghost planes that FLUXES performs
 // (sub:1390) so that the resident ghosts follow the reference exactly.
 // With CFL set (after stage 3), the boundary planes' CFL limits of the final
 // state are added to the reduction.
+// Column idx of the plane (npx*npy columns); the whole workgroup calls it.
 template <bool MAG, bool CFL>
-__global__ __launch_bounds__(256) void k_bcon(const Params p, const Metrics m, const double *__restrict__ qin,
-                                              double *__restrict__ q, DevScalars *__restrict__ sc) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bcon_column(const Params &p, const Metrics &m, const double *__restrict__ qin,
+                                            double *__restrict__ q, DevScalars *__restrict__ sc, int idx) {
     double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
     if (idx < p.npx * p.npy) {
         const long long fs = p.fsize, pl = p.plane;
@@ -524,6 +552,22 @@ __global__ __launch_bounds__(256) void k_bcon(const Params p, const Metrics m, c
     if (CFL) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
 }
 
+template <bool MAG, bool CFL>
+__global__ __launch_bounds__(256) void k_bcon(const Params p, const Metrics m, const double *__restrict__ qin,
+                                              double *__restrict__ q, DevScalars *__restrict__ sc) {
+    bcon_column<MAG, CFL>(p, m, qin, q, sc, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Ensemble: one workgroup per (box, run of 256 columns); register a of
+// every box was the stage input, 1-a the freshly written output.
+template <bool MAG, bool CFL>
+__global__ __launch_bounds__(256) void k_bcon_batch(const BoxDev *__restrict__ boxes,
+                                                    const ColumnItem *__restrict__ items, int a) {
+    const ColumnItem it = items[blockIdx.x];
+    const BoxDev &B = boxes[it.box];
+    bcon_column<MAG, CFL>(B.p, B.m, B.q[a], B.q[1 - a], B.sc, it.col0 + threadIdx.x);
+}
+
 // Stand-alone CFL reduction over planes 1..npz (first step after an upload).
 template <bool MAG>
 __global__ __launch_bounds__(256) void k_cfl(const Params p, const Metrics m, const double *__restrict__ q,
@@ -546,10 +590,10 @@ __global__ __launch_bounds__(256) void k_cfl(const Params p, const Metrics m, co
 }
 
 // End of a reduction: DT = SF*MIN(...), UMACH = SQRT(MAX(...)) (sub:761,
-// 868-872), then re-arm the keys.  With clock set this also closes the step
+// 868-872), then re-arm the keys.  A positive fixed_dt replaces the CFL DT.  With clock set this also closes the step
 // that just ran: NIT, TIMC, TIMT (sub:1171-1173) with the DT it used, which
 // is kept as dt_used / umach_used.
-__global__ void k_cfl_final(DevScalars *sc, double sf, int lmag, int clock) {
+__device__ __forceinline__ void cfl_final(DevScalars *sc, double sf, int lmag, int clock) {
     if (clock) {
         sc->nit = sc->nit + 1;
         sc->timc = sc->timc + sc->dt;
@@ -568,10 +612,19 @@ __global__ void k_cfl_final(DevScalars *sc, double sf, int lmag, int clock) {
         }
     }
     sc->umach = sqrt(mach);
-    sc->dt = sf * mn;
-    if (bad || sc->dt == INFINITY) sc->nonfinite |= 1;
+    const double dcfl = sf * mn;
+    sc->dt = sc->fixed_dt > 0.0 ? sc->fixed_dt : dcfl;
+    if (bad || dcfl == INFINITY) sc->nonfinite |= 1;
     sc->key_mach = dkey(0.0);
     for (int n = 0; n < 4; ++n) sc->key_min[n] = dkey(INFINITY);
+}
+
+__global__ void k_cfl_final(DevScalars *sc, double sf, int lmag, int clock) { cfl_final(sc, sf, lmag, clock); }
+
+// Ensemble: one thread per box.
+__global__ void k_cfl_final_batch(const BoxDev *__restrict__ boxes, int nbox, int clock) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nbox) cfl_final(boxes[b].sc, boxes[b].p.sf, boxes[b].p.lmag, clock);
 }
 
 // padded Fortran (npx+2, npy+2, npz+2) -> resident field (x/y ghosts dropped)
@@ -664,6 +717,64 @@ hipError_t launch_stage(int mode, const Params &p, const Metrics &m, const doubl
                         double *z, double *fout, DevScalars *sc, hipStream_t s) {
     if (p.tile_y == 8) return launch_stage_ty<8>(mode, p, m, qin, qout, z, fout, sc, s);
     return launch_stage_ty<4>(mode, p, m, qin, qout, z, fout, sc, s);
+}
+
+template <int TY, bool MAG, int MODE>
+static hipError_t launch_batch_t(const BoxDev *boxes, const StageItem *items, int nitems, int grid, int a,
+                                 hipStream_t s) {
+    constexpr int lds = stage_lds_bytes<TY, MAG>();
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_batch<TY, MAG, MODE>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_stage_batch<TY, MAG, MODE>), dim3(grid), dim3(TX * TY), lds, s, boxes, items, nitems, a);
+    return hipGetLastError();
+}
+
+template <int TY>
+static hipError_t launch_batch_ty(int mode, int lmag, const BoxDev *boxes, const StageItem *items, int nitems,
+                                  int grid, int a, hipStream_t s) {
+    if (lmag) {
+        if (mode == 1) return launch_batch_t<TY, true, 1>(boxes, items, nitems, grid, a, s);
+        if (mode == 2) return launch_batch_t<TY, true, 2>(boxes, items, nitems, grid, a, s);
+        return launch_batch_t<TY, true, 3>(boxes, items, nitems, grid, a, s);
+    }
+    if (mode == 1) return launch_batch_t<TY, false, 1>(boxes, items, nitems, grid, a, s);
+    if (mode == 2) return launch_batch_t<TY, false, 2>(boxes, items, nitems, grid, a, s);
+    return launch_batch_t<TY, false, 3>(boxes, items, nitems, grid, a, s);
+}
+
+hipError_t launch_stage_batch(int mode, int tile_y, int lmag, const BoxDev *boxes, const StageItem *items,
+                              int nitems, int grid, int a, hipStream_t s) {
+    if (tile_y == 8) return launch_batch_ty<8>(mode, lmag, boxes, items, nitems, grid, a, s);
+    return launch_batch_ty<4>(mode, lmag, boxes, items, nitems, grid, a, s);
+}
+
+hipError_t launch_bcon_batch(int lmag, const BoxDev *boxes, const ColumnItem *items, int nitems, int a, bool cfl,
+                             hipStream_t s) {
+    dim3 grid(nitems), block(256);
+    if (lmag) {
+        if (cfl) hipLaunchKernelGGL((k_bcon_batch<true, true>), grid, block, 0, s, boxes, items, a);
+        else hipLaunchKernelGGL((k_bcon_batch<true, false>), grid, block, 0, s, boxes, items, a);
+    } else {
+        if (cfl) hipLaunchKernelGGL((k_bcon_batch<false, true>), grid, block, 0, s, boxes, items, a);
+        else hipLaunchKernelGGL((k_bcon_batch<false, false>), grid, block, 0, s, boxes, items, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_final_batch(const BoxDev *boxes, int nbox, bool clock, hipStream_t s) {
+    hipLaunchKernelGGL(k_cfl_final_batch, dim3((nbox + 63) / 64), dim3(64), 0, s, boxes, nbox, clock ? 1 : 0);
+    return hipGetLastError();
+}
+
+int stage_lds(int tile_y, int lmag, int mixed) {
+    const int nf = lmag ? 8 : 5;
+    const int sp = (TX + 2) * (tile_y + 2);
+    return (2 * nf + 9) * sp * (mixed ? 4 : 8);
 }
 
 hipError_t launch_bcon(const Params &p, const Metrics &m, const double *qin, double *qout, DevScalars *sc,

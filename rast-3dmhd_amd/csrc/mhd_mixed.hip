@@ -5,11 +5,23 @@
 // side is evaluated in FP32 in perturbation form: F(bg + h) = F_bg(z) + F'(h),
 // where F_bg (nonzero only for the momentum z component and the temperature)
 // is the background's own discrete residual, formed once per plane in FP64.
-// The RK3 accumulation runs in FP64 in registers.  The second register D
-// (FP32) carries, per Wray's low-storage scheme, the zeta*dt*F increment of
-// the next stage plus the FP32 rounding residual of h, and after the last
-// stage the rounding residual alone, so that (h, D) is a compensated pair
-// and the storage rounding does not accumulate from step to step.
+//
+// Stored variables: RU, RV, RW, rho', P', BX, BY, BZ, where P' is the
+// pressure perturbation rho T - rho0 T0 = rho0 T' + T0 rho' + rho' T'.  In
+// low-Mach flow P' is a small residual of rho0 T' and T0 rho'; storing it
+// directly keeps the pressure gradient, which drives the horizontal flow,
+// accurate to the FP32 precision of P' itself, and T' = (P' - T0 rho')/rho
+// follows without cancellation.  (Storing T' instead leaves the horizontal
+// momentum several times 1e-6 off the FP64 result after 1000 steps.)
+//
+// The RK3 accumulation runs in FP64 in registers, on the same variables as
+// the FP64 form (T, not P): X = value + D + gamma dt F, where the value of
+// the temperature slot is T'(rho', P') evaluated in FP64.  The second
+// register D (FP32) carries, per Wray's low-storage scheme, the zeta*dt*F
+// increment of the next stage plus the rounding residual of the stored
+// value (for T: X_T - T'(new rho', new P')), and after the last stage the
+// residual alone, so that (h, D) is a compensated pair and the storage
+// rounding does not accumulate from step to step.
 // The CFL reduction, the boundary planes and the background are FP64.
 //
 // Traffic: every stage reads h (stencil) and D and writes h and D:
@@ -20,6 +32,9 @@
 namespace rmhd {
 
 constexpr int MTX = 64;
+
+// Arithmetic type of the perturbation right-hand side (storage stays FP32).
+typedef float rhs_t;
 
 template <int TY_>
 struct MTile {
@@ -32,9 +47,20 @@ struct MTile {
     static_assert(NHALO <= NT, "one halo point per thread at most");
 };
 
+// LDS: raw planes k, k+1 (2 x NF fields) and a three-plane ring of u, v, w, T'.
 template <int TY, bool MAG>
 constexpr int mixed_lds_bytes() {
-    return (2 * (MAG ? 8 : 5) + 9) * MTile<TY>::SP * (int)sizeof(float);
+    return (2 * (MAG ? 8 : 5) + 12) * MTile<TY>::SP * (int)sizeof(float);
+}
+
+// T' represented by a stored (rho', P') pair: (P' - T0 rho') / (rho0 + rho').
+__host__ __device__ __forceinline__ double tprime(double r0, double t0, float rh, float pp) {
+    return ((double)pp - t0 * (double)rh) / (r0 + (double)rh);
+}
+
+// P' of temperature perturbation tp at stored density perturbation rh.
+__host__ __device__ __forceinline__ double pprime(double r0, double t0, float rh, double tp) {
+    return r0 * tp + t0 * (double)rh + (double)rh * tp;
 }
 
 struct RawF {
@@ -47,31 +73,22 @@ __device__ __forceinline__ void load_rawf(const float *__restrict__ q, long long
     for (int f = 0; f < (MAG ? 8 : 5); ++f) r.v[f] = q[f * fs + off];
 }
 
-// MODE 1..3: RK3 stage s of the mixed form.
+// One tile column chunk of mixed stage s = MODE (1..3); see stage_tile of
+// the FP64 form for the tile organisation.
 template <int TY, bool MAG, int MODE>
-__global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metrics m, const Background bg,
-                                                     const float *__restrict__ hin, float *__restrict__ hout,
-                                                     float *__restrict__ dr, DevScalars *__restrict__ sc,
-                                                     int kchunk) {
+__device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, const Background &bg,
+                                             const float *__restrict__ hin, float *__restrict__ hout,
+                                             float *__restrict__ dr, DevScalars *__restrict__ sc, int i0, int j0,
+                                             int kbeg, int kend, float *ldsf) {
     using T = MTile<TY>;
     constexpr int SW = T::SW, SP = T::SP;
     constexpr int NF = MAG ? 8 : 5;
-    extern __shared__ float ldsf[];
     float *const s_cur = ldsf;
     float *const s_ring = ldsf + 2 * NF * SP;
 
     const int t = threadIdx.x;
     const int npx = p.npx, npy = p.npy, npz = p.npz;
     const long long fs = p.fsize, pl = p.plane;
-    const int ntx = (npx + MTX - 1) / MTX, nty = (npy + TY - 1) / TY;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int tz = bid / (ntx * nty);
-    const int rem = bid - tz * (ntx * nty);
-    const int tyb = rem / ntx, txb = rem - tyb * ntx;
-    const int i0 = txb * MTX, j0 = tyb * TY;
-    const int kbeg = 1 + tz * kchunk;
-    const int kend = min(npz, kbeg + kchunk - 1);
-    if (kbeg > npz) return;
 
     const int olx = t % MTX + 1, oly = t / MTX + 1;
     const int opt = oly * SW + olx;
@@ -92,7 +109,7 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
     const bool has_h = hpt >= 0;
 
     auto cur = [&](int k, int f, int pt) -> float & { return s_cur[((k & 1) * NF + f) * SP + pt]; };
-    auto ring = [&](int k, int c, int pt) -> float & { return s_ring[((k % 3) * 3 + c) * SP + pt]; };
+    auto ring = [&](int k, int c, int pt) -> float & { return s_ring[((k % 3) * 4 + c) * SP + pt]; };
     auto put_cur = [&](int k, int pt, const RawF &r) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) cur(k, f, pt) = r.v[f];
@@ -102,6 +119,7 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
         ring(k, 0, pt) = r.v[0] * ri;
         ring(k, 1, pt) = r.v[1] * ri;
         ring(k, 2, pt) = r.v[2] * ri;
+        ring(k, 3, pt) = (r.v[4] - bg.tt32[k] * r.v[3]) * ri;  // temperature perturbation
     };
 
     RawF rm_, rp_o, rp_h;
@@ -123,16 +141,16 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
         }
     }
 
-    const float HX = (float)p.hx, H2X = (float)p.h2x, HY = (float)p.hy, H2Y = (float)p.h2y;
-    const float HZ = (float)p.hz, H2Z = (float)p.h2z;
-    const float C43 = (float)p.c43, OCV = (float)p.ocv, ORE = (float)p.ore, ORM = (float)p.orm;
-    const float OBETA = (float)p.obeta, GRAV = (float)p.grav;
+    const rhs_t HX = (rhs_t)p.hx, H2X = (rhs_t)p.h2x, HY = (rhs_t)p.hy, H2Y = (rhs_t)p.h2y;
+    const rhs_t HZ = (rhs_t)p.hz, H2Z = (rhs_t)p.h2z;
+    const rhs_t C43 = (rhs_t)p.c43, OCV = (rhs_t)p.ocv, ORE = (rhs_t)p.ore, ORM = (rhs_t)p.orm;
+    const rhs_t OBETA = (rhs_t)p.obeta, GRAV = (rhs_t)p.grav;
     const int gix = wrapi(gi, npx) + 1, gjy = wrapi(gj, npy) + 1;
-    const float dx = (float)m.dxxdx[gix], d2x = (float)m.d2xxdx2[gix];
-    const float dy = (float)m.dyydy[gjy], d2y = (float)m.d2yydy2[gjy];
-    const float hxdx = HX * dx, hydy = HY * dy;
-    const float h2xdx2 = H2X * dx * dx, h2ydy2 = H2Y * dy * dy;
-    const float hxd2x = HX * d2x, hyd2y = HY * d2y;
+    const rhs_t dx = (rhs_t)m.dxxdx[gix], d2x = (rhs_t)m.d2xxdx2[gix];
+    const rhs_t dy = (rhs_t)m.dyydy[gjy], d2y = (rhs_t)m.d2yydy2[gjy];
+    const rhs_t hxdx = HX * dx, hydy = HY * dy;
+    const rhs_t h2xdx2 = H2X * dx * dx, h2ydy2 = H2Y * dy * dy;
+    const rhs_t hxd2x = HX * d2x, hyd2y = HY * d2y;
     const double dt = sc->dt;
     const double coefg = (MODE == 1 ? p.gam1 : MODE == 2 ? p.gam2 : p.gam3) * dt;
     const double coefz = (MODE == 1 ? p.zeta1 : p.zeta2) * dt;
@@ -155,42 +173,43 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
         __syncthreads();
         if (valid) {
             const bool top = (k == 1), bot = (k == npz);
-            const float dz = (float)m.dzzdz[k], d2z = (float)m.d2zzdz2[k];
-            const float rk = (float)m.rkapa[k], dkap = (float)m.dkapa[k];
-            const float hzdz = HZ * dz, hzd2z = HZ * d2z, h2zdz2 = H2Z * dz * dz;
-            const float r0 = bg.ro32[k], t0 = bg.tt32[k];
-            const float r0m = bg.ro32[k - 1], t0m = bg.tt32[k - 1];
-            const float r0p = bg.ro32[k + 1], t0p = bg.tt32[k + 1];
+            const rhs_t dz = (rhs_t)m.dzzdz[k], d2z = (rhs_t)m.d2zzdz2[k];
+            const rhs_t rk = (rhs_t)m.rkapa[k], dkap = (rhs_t)m.dkapa[k];
+            const rhs_t hzdz = HZ * dz, hzd2z = HZ * d2z, h2zdz2 = H2Z * dz * dz;
+            const rhs_t r0 = bg.ro32[k], t0 = bg.tt32[k];
 #define CUR(f, d) cur(k, f, opt + (d))
 #define NXT(f) cur(k + 1, f, opt)
 #define RNG(kp, c, d) ring(kp, c, opt + (d))
-            float q0[8];
+            float h0[8];
+            rhs_t q0[8];
 #pragma unroll
-            for (int f = 0; f < NF; ++f) q0[f] = CUR(f, 0);
-            const float rh0 = q0[3], th0 = q0[4];
-            const float tf0 = t0 + th0;                    // full temperature
-            const float rinv = 1.0f / (r0 + rh0);          // 1/rho
-            const float u0 = RNG(k, 0, 0), v0 = RNG(k, 1, 0), w0 = RNG(k, 2, 0);
-            const float um = RNG(k - 1, 0, 0), vm = RNG(k - 1, 1, 0), wm = RNG(k - 1, 2, 0);
-            const float up = RNG(k + 1, 0, 0), vp = RNG(k + 1, 1, 0), wp = RNG(k + 1, 2, 0);
-            const float uxp = RNG(k, 0, 1), uxm = RNG(k, 0, -1), uyp = RNG(k, 0, SW), uym = RNG(k, 0, -SW);
-            const float vxp = RNG(k, 1, 1), vxm = RNG(k, 1, -1), vyp = RNG(k, 1, SW), vym = RNG(k, 1, -SW);
-            const float wxp = RNG(k, 2, 1), wxm = RNG(k, 2, -1), wyp = RNG(k, 2, SW), wym = RNG(k, 2, -SW);
-            const float ru_m = rm_.v[0], rv_m = rm_.v[1], rw_m = rm_.v[2], rh_m = rm_.v[3], th_m = rm_.v[4];
-            const float ru_p = NXT(0), rv_p = NXT(1), rw_p = NXT(2), rh_p = NXT(3), th_p = NXT(4);
-            // pressure perturbation P' = rho0 T' + rho' (T0 + T')
-            auto pprime = [](float r0_, float t0_, float rh, float th) { return r0_ * th + rh * (t0_ + th); };
-            const float pm = pprime(r0m, t0m, rh_m, th_m), pp = pprime(r0p, t0p, rh_p, th_p);
-            const float pxm = pprime(r0, t0, CUR(3, -1), CUR(4, -1)), pxp = pprime(r0, t0, CUR(3, 1), CUR(4, 1));
-            const float pym = pprime(r0, t0, CUR(3, -SW), CUR(4, -SW)), pyp = pprime(r0, t0, CUR(3, SW), CUR(4, SW));
+            for (int f = 0; f < NF; ++f) {
+                h0[f] = CUR(f, 0);
+                q0[f] = h0[f];
+            }
+            const rhs_t rh0 = q0[3], th0 = RNG(k, 3, 0);
+            const rhs_t tf0 = t0 + th0;                    // full temperature
+            const rhs_t rinv = rhs_t(1) / (r0 + rh0);          // 1/rho
+            const rhs_t u0 = RNG(k, 0, 0), v0 = RNG(k, 1, 0), w0 = RNG(k, 2, 0);
+            const rhs_t um = RNG(k - 1, 0, 0), vm = RNG(k - 1, 1, 0), wm = RNG(k - 1, 2, 0);
+            const rhs_t up = RNG(k + 1, 0, 0), vp = RNG(k + 1, 1, 0), wp = RNG(k + 1, 2, 0);
+            const rhs_t uxp = RNG(k, 0, 1), uxm = RNG(k, 0, -1), uyp = RNG(k, 0, SW), uym = RNG(k, 0, -SW);
+            const rhs_t vxp = RNG(k, 1, 1), vxm = RNG(k, 1, -1), vyp = RNG(k, 1, SW), vym = RNG(k, 1, -SW);
+            const rhs_t wxp = RNG(k, 2, 1), wxm = RNG(k, 2, -1), wyp = RNG(k, 2, SW), wym = RNG(k, 2, -SW);
+            const rhs_t ru_m = rm_.v[0], rv_m = rm_.v[1], rw_m = rm_.v[2];
+            const rhs_t ru_p = NXT(0), rv_p = NXT(1), rw_p = NXT(2);
+            const rhs_t th_m = RNG(k - 1, 3, 0), th_p = RNG(k + 1, 3, 0);
+            // stored pressure perturbation at the six neighbours
+            const rhs_t pm = rm_.v[4], pp = NXT(4);
+            const rhs_t pxm = CUR(4, -1), pxp = CUR(4, 1), pym = CUR(4, -SW), pyp = CUR(4, SW);
 
-            float rwk2 = 0.f, wk2 = 0.f, bxk2 = 0.f, bxk3 = 0.f, byk2 = 0.f, byk3 = 0.f;
+            rhs_t rwk2 = rhs_t(0), wk2 = rhs_t(0), bxk2 = rhs_t(0), bxk3 = rhs_t(0), byk2 = rhs_t(0), byk3 = rhs_t(0);
             if (top || bot) {
                 const int ka = top ? k + 2 : k - 2;
                 const int kb = top ? k + 3 : k - 3;
                 const long long oa = (long long)ka * pl + ooff, ob = (long long)kb * pl + ooff;
                 rwk2 = hin[2 * fs + oa];
-                wk2 = rwk2 * (1.0f / (bg.ro32[ka] + hin[3 * fs + oa]));
+                wk2 = rwk2 * (rhs_t(1) / ((rhs_t)bg.ro32[ka] + (rhs_t)hin[3 * fs + oa]));
                 if (MAG) {
                     bxk2 = hin[5 * fs + oa];
                     byk2 = hin[6 * fs + oa];
@@ -200,14 +219,14 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
             }
 
             // continuity
-            float fr = (CUR(0, -1) - CUR(0, 1)) * hxdx - (CUR(1, SW) - CUR(1, -SW)) * hydy;
-            if (top) fr -= (4.0f * rw_p - rwk2) * hzdz;
-            else if (bot) fr -= (rwk2 - 4.0f * rw_m) * hzdz;
+            rhs_t fr = (CUR(0, -1) - CUR(0, 1)) * hxdx - (CUR(1, SW) - CUR(1, -SW)) * hydy;
+            if (top) fr -= (rhs_t(4) * rw_p - rwk2) * hzdz;
+            else if (bot) fr -= (rwk2 - rhs_t(4) * rw_m) * hzdz;
             else fr -= (rw_p - rw_m) * hzdz;
             // buoyancy and pressure gradient of the perturbation
-            float fw = GRAV * rh0 - (pp - pm) * hzdz;
-            float fv = (pym - pyp) * hydy;
-            float fu = (pxm - pxp) * hxdx;
+            rhs_t fw = GRAV * rh0 - (pp - pm) * hzdz;
+            rhs_t fv = (pym - pyp) * hydy;
+            rhs_t fu = (pxm - pxp) * hxdx;
             // momentum advection
             fu -= (CUR(0, 1) * uxp - CUR(0, -1) * uxm) * hxdx + (CUR(0, SW) * vyp - CUR(0, -SW) * vym) * hydy
                   + (ru_p * wp - ru_m * wm) * hzdz;
@@ -216,35 +235,35 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
             fw -= (CUR(2, 1) * uxp - CUR(2, -1) * uxm) * hxdx + (CUR(2, SW) * vyp - CUR(2, -SW) * vym) * hydy
                   + (rw_p * wp - rw_m * wm) * hzdz;
             // temperature advection, the background gradient included
-            const float txp = CUR(4, 1), txm = CUR(4, -1), typ = CUR(4, SW), tym = CUR(4, -SW);
-            const float dtz = (th_p - th_m) * hzdz;
-            float ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + bg.dt0dz32[k]));
+            const rhs_t txp = RNG(k, 3, 1), txm = RNG(k, 3, -1), typ = RNG(k, 3, SW), tym = RNG(k, 3, -SW);
+            const rhs_t dtz = (th_p - th_m) * hzdz;
+            rhs_t ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + bg.dt0dz32[k]));
             // conduction of T': (1/rho) c [kappa lap T' + kappa' dT'/dz]; plus the
             // change of the background's conduction with 1/rho: -rho'/(rho rho0) S0
             {
-                const float lap = (txp - txm) * hxd2x + (txp - 2.0f * th0 + txm) * h2xdx2
-                                  + (typ - tym) * hyd2y + (typ - 2.0f * th0 + tym) * h2ydy2
-                                  + (th_p - th_m) * hzd2z + (th_p - 2.0f * th0 + th_m) * h2zdz2;
-                ft += rinv * (float)p.t_cond * (rk * lap + dkap * dtz) - rh0 * rinv * bg.rinv0_32[k] * bg.s0_32[k];
+                const rhs_t lap = (txp - txm) * hxd2x + (txp - rhs_t(2) * th0 + txm) * h2xdx2
+                                  + (typ - tym) * hyd2y + (typ - rhs_t(2) * th0 + tym) * h2ydy2
+                                  + (th_p - th_m) * hzd2z + (th_p - rhs_t(2) * th0 + th_m) * h2zdz2;
+                ft += rinv * (rhs_t)p.t_cond * (rk * lap + dkap * dtz) - rh0 * rinv * bg.rinv0_32[k] * bg.s0_32[k];
             }
             // viscous Laplacians
-            fu += ORE * (C43 * ((uxp - uxm) * hxd2x + (uxp - 2.0f * u0 + uxm) * h2xdx2)
-                         + (uyp - uym) * hyd2y + (uyp - 2.0f * u0 + uym) * h2ydy2
-                         + (up - um) * hzd2z + (up - 2.0f * u0 + um) * h2zdz2);
-            fv += ORE * ((vxp - vxm) * hxd2x + (vxp - 2.0f * v0 + vxm) * h2xdx2
-                         + C43 * ((vyp - vym) * hyd2y + (vyp - 2.0f * v0 + vym) * h2ydy2)
-                         + (vp - vm) * hzd2z + (vp - 2.0f * v0 + vm) * h2zdz2);
-            fw += ORE * ((wxp - wxm) * hxd2x + (wxp - 2.0f * w0 + wxm) * h2xdx2
-                         + (wyp - wym) * hyd2y + (wyp - 2.0f * w0 + wym) * h2ydy2
-                         + C43 * ((wp - wm) * hzd2z + (wp - 2.0f * w0 + wm) * h2zdz2));
+            fu += ORE * (C43 * ((uxp - uxm) * hxd2x + (uxp - rhs_t(2) * u0 + uxm) * h2xdx2)
+                         + (uyp - uym) * hyd2y + (uyp - rhs_t(2) * u0 + uym) * h2ydy2
+                         + (up - um) * hzd2z + (up - rhs_t(2) * u0 + um) * h2zdz2);
+            fv += ORE * ((vxp - vxm) * hxd2x + (vxp - rhs_t(2) * v0 + vxm) * h2xdx2
+                         + C43 * ((vyp - vym) * hyd2y + (vyp - rhs_t(2) * v0 + vym) * h2ydy2)
+                         + (vp - vm) * hzd2z + (vp - rhs_t(2) * v0 + vm) * h2zdz2);
+            fw += ORE * ((wxp - wxm) * hxd2x + (wxp - rhs_t(2) * w0 + wxm) * h2xdx2
+                         + (wyp - wym) * hyd2y + (wyp - rhs_t(2) * w0 + wym) * h2ydy2
+                         + C43 * ((wp - wm) * hzd2z + (wp - rhs_t(2) * w0 + wm) * h2zdz2));
             // heating and grad div u cross terms
-            const float tv = (float)p.t_visc, tv43 = (float)p.t_visc43, tcr = (float)p.t_cross;
-            const float dxu = (uxp - uxm) * hxdx, dxv = (vxp - vxm) * hxdx, dxw = (wxp - wxm) * hxdx;
-            const float dyu = (uyp - uym) * hydy, dyv = (vyp - vym) * hydy, dyw = (wyp - wym) * hydy;
-            const float dzu = (up - um) * hzdz, dzv = (vp - vm) * hzdz, dzw = (wp - wm) * hzdz;
-            ft += (C43 * dxu * dxu + dxv * dxv + dxw * dxw + dyu * dyu + 2.0f * dyu * dxv) * rinv * tv
+            const rhs_t tv = (rhs_t)p.t_visc, tv43 = (rhs_t)p.t_visc43, tcr = (rhs_t)p.t_cross;
+            const rhs_t dxu = (uxp - uxm) * hxdx, dxv = (vxp - vxm) * hxdx, dxw = (wxp - wxm) * hxdx;
+            const rhs_t dyu = (uyp - uym) * hydy, dyv = (vyp - vym) * hydy, dyw = (wyp - wym) * hydy;
+            const rhs_t dzu = (up - um) * hzdz, dzv = (vp - vm) * hzdz, dzw = (wp - wm) * hzdz;
+            ft += (C43 * dxu * dxu + dxv * dxv + dxw * dxw + dyu * dyu + rhs_t(2) * dyu * dxv) * rinv * tv
                   + (dyv * dyv + dzw * dzw - dxu * dzw - dxu * dyv - dyv * dzw) * rinv * tv43
-                  + (dzu * dzu + dzv * dzv + dyw * dyw + 2.0f * dzv * dyw + 2.0f * dzu * dxw) * rinv * tv
+                  + (dzu * dzu + dzv * dzv + dyw * dyw + rhs_t(2) * dzv * dyw + rhs_t(2) * dzu * dxw) * rinv * tv
                   - OCV * (dxu + dyv + dzw) * tf0;
             fu += tcr * (((RNG(k, 1, SW + 1) - RNG(k, 1, SW - 1)) - (RNG(k, 1, -SW + 1) - RNG(k, 1, -SW - 1))) * hxdx * hydy
                          + ((RNG(k + 1, 2, 1) - RNG(k - 1, 2, 1)) - (RNG(k + 1, 2, -1) - RNG(k - 1, 2, -1))) * hzdz * hxdx);
@@ -254,26 +273,26 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
                          + ((RNG(k + 1, 1, SW) - RNG(k - 1, 1, SW)) - (RNG(k + 1, 1, -SW) - RNG(k - 1, 1, -SW))) * hzdz * hydy);
             // Coriolis
             if (p.lrot) {
-                const float omx = (float)p.omx, omz = (float)p.omz;
+                const rhs_t omx = (rhs_t)p.omx, omz = (rhs_t)p.omz;
                 fu += omz * q0[1];
                 fv += -omz * q0[0] + omx * q0[2];
                 fw -= omx * q0[1];
             }
             // MHD
-            float w1 = 0.f, w2 = 0.f, w3 = 0.f;
+            rhs_t w1 = rhs_t(0), w2 = rhs_t(0), w3 = rhs_t(0);
             if (MAG) {
-                const float bx0 = q0[5], by0 = q0[6], bz0 = q0[7];
-                const float bxm = rm_.v[5], bym = rm_.v[6], bzm = rm_.v[7];
-                const float bxp = NXT(5), byp = NXT(6), bzp = NXT(7);
-                const float bxxp = CUR(5, 1), bxxm = CUR(5, -1), bxyp = CUR(5, SW), bxym = CUR(5, -SW);
-                const float byxp = CUR(6, 1), byxm = CUR(6, -1), byyp = CUR(6, SW), byym = CUR(6, -SW);
-                const float bzxp = CUR(7, 1), bzxm = CUR(7, -1), bzyp = CUR(7, SW), bzym = CUR(7, -SW);
-                float dzw1 = dzw;
-                if (top) dzw1 = (4.0f * wp - wk2) * hzdz;
-                else if (bot) dzw1 = (wk2 - 4.0f * wm) * hzdz;
-                const float dxbx = (bxxp - bxxm) * hxdx, dxby = (byxp - byxm) * hxdx, dxbz = (bzxp - bzxm) * hxdx;
-                const float dybx = (bxyp - bxym) * hydy, dyby = (byyp - byym) * hydy, dybz = (bzyp - bzym) * hydy;
-                const float dzbx = (bxp - bxm) * hzdz, dzby = (byp - bym) * hzdz, dzbz = (bzp - bzm) * hzdz;
+                const rhs_t bx0 = q0[5], by0 = q0[6], bz0 = q0[7];
+                const rhs_t bxm = rm_.v[5], bym = rm_.v[6], bzm = rm_.v[7];
+                const rhs_t bxp = NXT(5), byp = NXT(6), bzp = NXT(7);
+                const rhs_t bxxp = CUR(5, 1), bxxm = CUR(5, -1), bxyp = CUR(5, SW), bxym = CUR(5, -SW);
+                const rhs_t byxp = CUR(6, 1), byxm = CUR(6, -1), byyp = CUR(6, SW), byym = CUR(6, -SW);
+                const rhs_t bzxp = CUR(7, 1), bzxm = CUR(7, -1), bzyp = CUR(7, SW), bzym = CUR(7, -SW);
+                rhs_t dzw1 = dzw;
+                if (top) dzw1 = (rhs_t(4) * wp - wk2) * hzdz;
+                else if (bot) dzw1 = (wk2 - rhs_t(4) * wm) * hzdz;
+                const rhs_t dxbx = (bxxp - bxxm) * hxdx, dxby = (byxp - byxm) * hxdx, dxbz = (bzxp - bzxm) * hxdx;
+                const rhs_t dybx = (bxyp - bxym) * hydy, dyby = (byyp - byym) * hydy, dybz = (bzyp - bzym) * hydy;
+                const rhs_t dzbx = (bxp - bxm) * hzdz, dzby = (byp - bym) * hzdz, dzbz = (bzp - bzm) * hzdz;
                 // induction (B.grad) u - B div u - (u.grad) B
                 w1 = by0 * dyu - bx0 * dyv + bz0 * dzu - bx0 * dzw1 - u0 * dxbx - v0 * dybx - w0 * dzbx;
                 w2 = bx0 * dxv - by0 * dxu + bz0 * dzv - by0 * dzw1 - u0 * dxby - v0 * dyby - w0 * dzby;
@@ -283,44 +302,51 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
                 fv += OBETA * (bx0 * (dxby - dybx) + bz0 * (dzby - dybz));
                 fw += OBETA * (bx0 * dxbz - bx0 * dzbx - by0 * (dzby - dybz));
                 // Ohmic heating |J|^2 / rho
-                ft += ((dybx - dxby) * (dybx - dxby) + dxbz * dxbz + dzbx * dzbx - 2.0f * dzbx * dxbz
-                       + (dzby - dybz) * (dzby - dybz)) * rinv * (float)p.t_ohm;
+                ft += ((dybx - dxby) * (dybx - dxby) + dxbz * dxbz + dzbx * dzbx - rhs_t(2) * dzbx * dxbz
+                       + (dzby - dybz) * (dzby - dybz)) * rinv * (rhs_t)p.t_ohm;
                 // resistive diffusion, one-sided in z on the boundary planes for BX, BY
-                float d2zbx, d2zby;
+                rhs_t d2zbx, d2zby;
                 if (top) {
-                    d2zbx = (-3.0f * bx0 + 4.0f * bxp - bxk2) * hzd2z + (2.0f * bx0 - 5.0f * bxp + 4.0f * bxk2 - bxk3) * h2zdz2;
-                    d2zby = (-3.0f * by0 + 4.0f * byp - byk2) * hzd2z + (2.0f * by0 - 5.0f * byp + 4.0f * byk2 - byk3) * h2zdz2;
+                    d2zbx = (-rhs_t(3) * bx0 + rhs_t(4) * bxp - bxk2) * hzd2z + (rhs_t(2) * bx0 - rhs_t(5) * bxp + rhs_t(4) * bxk2 - bxk3) * h2zdz2;
+                    d2zby = (-rhs_t(3) * by0 + rhs_t(4) * byp - byk2) * hzd2z + (rhs_t(2) * by0 - rhs_t(5) * byp + rhs_t(4) * byk2 - byk3) * h2zdz2;
                 } else if (bot) {
-                    d2zbx = (3.0f * bx0 - 4.0f * bxm + bxk2) * hzd2z + (2.0f * bx0 - 5.0f * bxm + 4.0f * bxk2 - bxk3) * h2zdz2;
-                    d2zby = (3.0f * by0 - 4.0f * bym + byk2) * hzd2z + (2.0f * by0 - 5.0f * bym + 4.0f * byk2 - byk3) * h2zdz2;
+                    d2zbx = (rhs_t(3) * bx0 - rhs_t(4) * bxm + bxk2) * hzd2z + (rhs_t(2) * bx0 - rhs_t(5) * bxm + rhs_t(4) * bxk2 - bxk3) * h2zdz2;
+                    d2zby = (rhs_t(3) * by0 - rhs_t(4) * bym + byk2) * hzd2z + (rhs_t(2) * by0 - rhs_t(5) * bym + rhs_t(4) * byk2 - byk3) * h2zdz2;
                 } else {
-                    d2zbx = (bxp - bxm) * hzd2z + (bxp - 2.0f * bx0 + bxm) * h2zdz2;
-                    d2zby = (byp - bym) * hzd2z + (byp - 2.0f * by0 + bym) * h2zdz2;
+                    d2zbx = (bxp - bxm) * hzd2z + (bxp - rhs_t(2) * bx0 + bxm) * h2zdz2;
+                    d2zby = (byp - bym) * hzd2z + (byp - rhs_t(2) * by0 + bym) * h2zdz2;
                 }
-                const float d2zbz = (bzp - bzm) * hzd2z + (bzp - 2.0f * bz0 + bzm) * h2zdz2;
-                w1 += ORM * ((bxxp - bxxm) * hxd2x + (bxxp - 2.0f * bx0 + bxxm) * h2xdx2
-                             + (bxyp - bxym) * hyd2y + (bxyp - 2.0f * bx0 + bxym) * h2ydy2 + d2zbx);
-                w2 += ORM * ((byxp - byxm) * hxd2x + (byxp - 2.0f * by0 + byxm) * h2xdx2
-                             + (byyp - byym) * hyd2y + (byyp - 2.0f * by0 + byym) * h2ydy2 + d2zby);
-                w3 += ORM * ((bzxp - bzxm) * hxd2x + (bzxp - 2.0f * bz0 + bzxm) * h2xdx2
-                             + (bzyp - bzym) * hyd2y + (bzyp - 2.0f * bz0 + bzym) * h2ydy2 + d2zbz);
+                const rhs_t d2zbz = (bzp - bzm) * hzd2z + (bzp - rhs_t(2) * bz0 + bzm) * h2zdz2;
+                w1 += ORM * ((bxxp - bxxm) * hxd2x + (bxxp - rhs_t(2) * bx0 + bxxm) * h2xdx2
+                             + (bxyp - bxym) * hyd2y + (bxyp - rhs_t(2) * bx0 + bxym) * h2ydy2 + d2zbx);
+                w2 += ORM * ((byxp - byxm) * hxd2x + (byxp - rhs_t(2) * by0 + byxm) * h2xdx2
+                             + (byyp - byym) * hyd2y + (byyp - rhs_t(2) * by0 + byym) * h2ydy2 + d2zby);
+                w3 += ORM * ((bzxp - bzxm) * hxd2x + (bzxp - rhs_t(2) * bz0 + bzxm) * h2xdx2
+                             + (bzyp - bzym) * hyd2y + (bzyp - rhs_t(2) * bz0 + bzym) * h2ydy2 + d2zbz);
             }
 #undef CUR
 #undef NXT
 #undef RNG
-            // FP64 accumulation: F = F_bg(z) + F', X = h + D + gamma dt F,
-            // store h' = fl32(X) and carry the residual (+ zeta dt F) in D
+            // FP64 accumulation: F = F_bg(z) + F', X = value + D + gamma dt F
+            // (the temperature slot's value is T'(rho', P')); store the new
+            // FP32 values and carry the residual (+ zeta dt F) in D.
             double Fd[8] = {fu, fv, fw, fr, ft, w1, w2, w3};
             Fd[2] += bg.fwbg[k];
             Fd[4] += bg.ftbg[k];
+            const double r0d = bg.ro[k], t0d = bg.tt[k];
             double xq[8];
+            float hn[8];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                const double X = ((double)q0[f] + (double)dk_[f]) + coefg * Fd[f];
-                const float hn = (float)X;
-                const double r = X - (double)hn;
-                xq[f] = X;
-                __builtin_nontemporal_store(hn, &hout[f * fs + o]);
+                const double v = f == 4 ? tprime(r0d, t0d, h0[3], h0[4]) : (double)h0[f];
+                xq[f] = (v + (double)dk_[f]) + coefg * Fd[f];
+                if (f != 4) hn[f] = (float)xq[f];
+            }
+            hn[4] = (float)pprime(r0d, t0d, hn[3], xq[4]);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const double r = xq[f] - (f == 4 ? tprime(r0d, t0d, hn[3], hn[4]) : (double)hn[f]);
+                __builtin_nontemporal_store(hn[f], &hout[f * fs + o]);
                 const float dn = (MODE < 3) ? (float)(r + coefz * Fd[f]) : (float)r;
                 __builtin_nontemporal_store(dn, &dr[f * fs + o]);
             }
@@ -333,7 +359,7 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
                 cfl_point<MAG>(p, full, c_ddx, c_ddy, m.ddz[k], m.rkapa[k], c_mach, c_w1, c_w2, c_w3, c_w4);
             }
 #pragma unroll
-            for (int f = 0; f < NF; ++f) rm_.v[f] = q0[f];
+            for (int f = 0; f < NF; ++f) rm_.v[f] = h0[f];
         }
         __syncthreads();
         if (pre) {
@@ -348,14 +374,46 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
     if (MODE == 3) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
 }
 
-// BCON of the mixed form, in FP64 on the full state bg + h (+ D after the
-// last stage, where D holds only the rounding residual).  New boundary
-// values are split again into h and the residual D.
+template <int TY, bool MAG, int MODE>
+__global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metrics m, const Background bg,
+                                                     const float *__restrict__ hin, float *__restrict__ hout,
+                                                     float *__restrict__ dr, DevScalars *__restrict__ sc,
+                                                     int kchunk) {
+    extern __shared__ float ldsf[];
+    const int ntx = (p.npx + MTX - 1) / MTX, nty = (p.npy + TY - 1) / TY;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int tz = bid / (ntx * nty);
+    const int rem = bid - tz * (ntx * nty);
+    const int tyb = rem / ntx, txb = rem - tyb * ntx;
+    const int kbeg = 1 + tz * kchunk;
+    const int kend = min(p.npz, kbeg + kchunk - 1);
+    if (kbeg > p.npz) return;
+    stage_tile_m<TY, MAG, MODE>(p, m, bg, hin, hout, dr, sc, txb * MTX, tyb * TY, kbeg, kend, ldsf);
+}
+
+// Ensemble: persistent grid over the work items of all boxes (static stride).
+template <int TY, bool MAG, int MODE>
+__global__ __launch_bounds__(MTX *TY) void k_stage_m_batch(const BoxDev *__restrict__ boxes,
+                                                           const StageItem *__restrict__ items, int nitems, int a) {
+    extern __shared__ float ldsf[];
+    const int slot = xcd_remap(blockIdx.x, gridDim.x);
+    for (int w = slot; w < nitems; w += gridDim.x) {
+        const StageItem it = items[w];
+        const BoxDev &B = boxes[it.box];
+        __syncthreads();  // the previous item's last LDS reads are done
+        stage_tile_m<TY, MAG, MODE>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.sc, it.i0, it.j0, it.kbeg, it.kend,
+                                    ldsf);
+    }
+}
+
+// BCON of the mixed form, in FP64 on the full state bg + value (+ D after
+// the last stage, where D holds only the rounding residual).  New boundary
+// values are split again into stored values and the residual D; a new
+// boundary temperature is stored as the P' of the plane's stored rho'.
 template <bool MAG, bool LAST>
-__global__ __launch_bounds__(256) void k_bcon_m(const Params p, const Metrics m, const Background bg,
-                                                float *__restrict__ h, float *__restrict__ dr,
-                                                DevScalars *__restrict__ sc) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bcon_column_m(const Params &p, const Metrics &m, const Background &bg,
+                                              float *__restrict__ h, float *__restrict__ dr,
+                                              DevScalars *__restrict__ sc, int idx) {
     double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
     if (idx < p.npx * p.npy) {
         const long long fs = p.fsize, pl = p.plane;
@@ -363,16 +421,23 @@ __global__ __launch_bounds__(256) void k_bcon_m(const Params p, const Metrics m,
         const double C13 = p.c13, C23 = p.c23, C43 = p.c43;
         auto full = [&](int f, int k) -> double {
             const long long o = f * fs + (long long)k * pl + idx;
-            double v = (double)h[o];
+            double v = f == 4 ? tprime(bg.ro[k], bg.tt[k], h[o - fs], h[o]) : (double)h[o];
             if (LAST) v += (double)dr[o];
             if (f == 3) v += bg.ro[k];
             if (f == 4) v += bg.tt[k];
             return v;
         };
         auto put = [&](int f, int k, double v) {
-            if (f == 3) v -= bg.ro[k];
-            if (f == 4) v -= bg.tt[k];
             const long long o = f * fs + (long long)k * pl + idx;
+            if (f == 4) {
+                const double tp = v - bg.tt[k];
+                const float rh = h[o - fs];
+                const float hp = (float)pprime(bg.ro[k], bg.tt[k], rh, tp);
+                h[o] = hp;
+                dr[o] = (float)(tp - tprime(bg.ro[k], bg.tt[k], rh, hp));
+                return;
+            }
+            if (f == 3) v -= bg.ro[k];
             const float hn = (float)v;
             h[o] = hn;
             dr[o] = (float)(v - (double)hn);
@@ -409,6 +474,22 @@ __global__ __launch_bounds__(256) void k_bcon_m(const Params p, const Metrics m,
     if (LAST) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
 }
 
+template <bool MAG, bool LAST>
+__global__ __launch_bounds__(256) void k_bcon_m(const Params p, const Metrics m, const Background bg,
+                                                float *__restrict__ h, float *__restrict__ dr,
+                                                DevScalars *__restrict__ sc) {
+    bcon_column_m<MAG, LAST>(p, m, bg, h, dr, sc, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Ensemble: one workgroup per (box, run of 256 columns) of register 1-a.
+template <bool MAG, bool LAST>
+__global__ __launch_bounds__(256) void k_bcon_m_batch(const BoxDev *__restrict__ boxes,
+                                                      const ColumnItem *__restrict__ items, int a) {
+    const ColumnItem it = items[blockIdx.x];
+    const BoxDev &B = boxes[it.box];
+    bcon_column_m<MAG, LAST>(B.p, B.m, B.bg, B.h[1 - a], B.dr, B.sc, it.col0 + threadIdx.x);
+}
+
 // Stand-alone CFL of the mixed state (first step after an upload).
 template <bool MAG>
 __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, const Background bg,
@@ -424,7 +505,10 @@ __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, 
         const long long r2 = idx - (long long)(k - 1) * pl;
         const int j = (int)(r2 / p.npx), i = (int)(r2 - (long long)j * p.npx);
         double v[8];
-        for (int f = 0; f < (MAG ? 8 : 5); ++f) v[f] = (double)h[f * fs + o] + (double)dr[f * fs + o];
+        for (int f = 0; f < (MAG ? 8 : 5); ++f) {
+            const double x = f == 4 ? tprime(bg.ro[k], bg.tt[k], h[3 * fs + o], h[4 * fs + o]) : (double)h[f * fs + o];
+            v[f] = x + (double)dr[f * fs + o];
+        }
         v[3] += bg.ro[k];
         v[4] += bg.tt[k];
         cfl_point<MAG>(p, v, m.ddx[i + 1], m.ddy[j + 1], m.ddz[k], m.rkapa[k], mach, w1, w2, w3, w4);
@@ -432,38 +516,51 @@ __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, 
     cfl_reduce(mach, w1, w2, w3, w4, p.lmag, sc);
 }
 
-// padded FP64 field -> (h, D) with D the FP32 rounding residual
-__global__ void k_pack_m(const Params p, const double *__restrict__ padded, const double *__restrict__ bgcol,
-                         float *__restrict__ h, float *__restrict__ dr) {
-    const long long n = p.fsize;
+// padded FP64 field f -> (stored value, D = rounding residual) in the
+// registers hreg, dreg (all fields); the temperature is stored as the P' of
+// the already packed rho' of the same register.
+__global__ void k_pack_m(const Params p, const double *__restrict__ padded, int field, const Background bg,
+                         float *__restrict__ hreg, float *__restrict__ dreg) {
+    const long long n = p.fsize, fs = p.fsize;
     const int nxp = p.npx + 2, nyp = p.npy + 2;
+    float *const h = hreg + field * fs;
+    float *const dr = dreg + field * fs;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long long)gridDim.x * blockDim.x) {
         const long long k = idx / p.plane;
         const long long r = idx - k * p.plane;
         const int j = (int)(r / p.npx), i = (int)(r - (long long)j * p.npx);
         double v = padded[(k * nyp + (j + 1)) * nxp + (i + 1)];
-        if (bgcol) v -= bgcol[k];
+        if (field == 4) {
+            const double tp = v - bg.tt[k];
+            const float rh = hreg[3 * fs + idx];
+            const float hp = (float)pprime(bg.ro[k], bg.tt[k], rh, tp);
+            h[idx] = hp;
+            dr[idx] = (float)(tp - tprime(bg.ro[k], bg.tt[k], rh, hp));
+            continue;
+        }
+        if (field == 3) v -= bg.ro[k];
         const float hn = (float)v;
         h[idx] = hn;
         dr[idx] = (float)(v - (double)hn);
     }
 }
 
-// (h, D) -> padded FP64 field with periodic ghosts
-__global__ void k_unpack_m(const Params p, const float *__restrict__ h, const float *__restrict__ dr,
-                           const double *__restrict__ bgcol, double *__restrict__ padded) {
+// field f of (stored value, D) -> padded FP64 field with periodic ghosts
+__global__ void k_unpack_m(const Params p, const float *__restrict__ hreg, const float *__restrict__ dreg, int field,
+                           const Background bg, double *__restrict__ padded) {
     const int nxp = p.npx + 2, nyp = p.npy + 2, nzp = p.npz + 2;
-    const long long n = (long long)nxp * nyp * nzp;
+    const long long n = (long long)nxp * nyp * nzp, fs = p.fsize;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long long)gridDim.x * blockDim.x) {
         const long long k = idx / ((long long)nxp * nyp);
         const long long r = idx - k * nxp * nyp;
         const int J = (int)(r / nxp), I = (int)(r - (long long)J * nxp);
         const long long o = k * p.plane + (long long)wrapi(J - 1, p.npy) * p.npx + wrapi(I - 1, p.npx);
-        double v = (double)h[o] + (double)dr[o];
-        if (bgcol) v += bgcol[k];
-        padded[idx] = v;
+        double v;
+        if (field == 4) v = bg.tt[k] + tprime(bg.ro[k], bg.tt[k], hreg[3 * fs + o], hreg[4 * fs + o]);
+        else v = (double)hreg[field * fs + o] + (field == 3 ? bg.ro[k] : 0.0);
+        padded[idx] = v + (double)dreg[field * fs + o];
     }
 }
 
@@ -543,20 +640,68 @@ hipError_t launch_cfl_mixed(const Params &p, const Metrics &m, const Background 
     return hipGetLastError();
 }
 
-hipError_t launch_pack_mixed(const Params &p, const double *padded, const double *bgcol, float *h, float *dr,
-                             hipStream_t s) {
+hipError_t launch_pack_mixed(const Params &p, const double *padded, int field, const Background &bg, float *hreg,
+                             float *dreg, hipStream_t s) {
     long long nb = (p.fsize + 255) / 256;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(k_pack_m, dim3((unsigned)nb), dim3(256), 0, s, p, padded, bgcol, h, dr);
+    hipLaunchKernelGGL(k_pack_m, dim3((unsigned)nb), dim3(256), 0, s, p, padded, field, bg, hreg, dreg);
     return hipGetLastError();
 }
 
-hipError_t launch_unpack_mixed(const Params &p, const float *h, const float *dr, const double *bgcol,
+hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const float *dreg, int field, const Background &bg,
                                double *padded, hipStream_t s) {
     const long long n = (long long)(p.npx + 2) * (p.npy + 2) * (p.npz + 2);
     long long nb = (n + 255) / 256;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(k_unpack_m, dim3((unsigned)nb), dim3(256), 0, s, p, h, dr, bgcol, padded);
+    hipLaunchKernelGGL(k_unpack_m, dim3((unsigned)nb), dim3(256), 0, s, p, hreg, dreg, field, bg, padded);
+    return hipGetLastError();
+}
+
+template <int TY, bool MAG, int MODE>
+static hipError_t launch_mb(const BoxDev *boxes, const StageItem *items, int nitems, int grid, int a,
+                            hipStream_t s) {
+    constexpr int lds = mixed_lds_bytes<TY, MAG>();
+    static bool attr = false;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_m_batch<TY, MAG, MODE>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_stage_m_batch<TY, MAG, MODE>), dim3(grid), dim3(MTX * TY), lds, s, boxes, items, nitems,
+                       a);
+    return hipGetLastError();
+}
+
+template <int TY>
+static hipError_t launch_mb_ty(int mode, int lmag, const BoxDev *boxes, const StageItem *items, int nitems,
+                               int grid, int a, hipStream_t s) {
+    if (lmag) {
+        if (mode == 1) return launch_mb<TY, true, 1>(boxes, items, nitems, grid, a, s);
+        if (mode == 2) return launch_mb<TY, true, 2>(boxes, items, nitems, grid, a, s);
+        return launch_mb<TY, true, 3>(boxes, items, nitems, grid, a, s);
+    }
+    if (mode == 1) return launch_mb<TY, false, 1>(boxes, items, nitems, grid, a, s);
+    if (mode == 2) return launch_mb<TY, false, 2>(boxes, items, nitems, grid, a, s);
+    return launch_mb<TY, false, 3>(boxes, items, nitems, grid, a, s);
+}
+
+hipError_t launch_stage_batch_mixed(int mode, int tile_y, int lmag, const BoxDev *boxes, const StageItem *items,
+                                    int nitems, int grid, int a, hipStream_t s) {
+    if (tile_y == 8) return launch_mb_ty<8>(mode, lmag, boxes, items, nitems, grid, a, s);
+    return launch_mb_ty<4>(mode, lmag, boxes, items, nitems, grid, a, s);
+}
+
+hipError_t launch_bcon_batch_mixed(int lmag, const BoxDev *boxes, const ColumnItem *items, int nitems, int a,
+                                   bool last, hipStream_t s) {
+    dim3 grid(nitems), block(256);
+    if (lmag) {
+        if (last) hipLaunchKernelGGL((k_bcon_m_batch<true, true>), grid, block, 0, s, boxes, items, a);
+        else hipLaunchKernelGGL((k_bcon_m_batch<true, false>), grid, block, 0, s, boxes, items, a);
+    } else {
+        if (last) hipLaunchKernelGGL((k_bcon_m_batch<false, true>), grid, block, 0, s, boxes, items, a);
+        else hipLaunchKernelGGL((k_bcon_m_batch<false, false>), grid, block, 0, s, boxes, items, a);
+    }
     return hipGetLastError();
 }
 

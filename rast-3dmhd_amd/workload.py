@@ -53,6 +53,29 @@ def stratified_box(npx: int, npy: int, npz: int, *, seed: int = 20260):
                           tilt_deg=0.0, ampt=1e-4, kmax=4, ampb=1e-5, ngrid=3)
 
 
+def ensemble_specs(n: int = 64, seed: int = 4242):
+    """Box specifications of a random ensemble: horizontal sizes uniform over
+    the multiples of 8 in 32..96, vertical sizes uniform in 32..64, Ra
+    log-uniform in 1e4..1e6, Taylor log-uniform in 1e2..1e6, density
+    contrast uniform in 3..30; box b gets the initial-condition seed
+    seed + 1 + b.  Drawn from numpy's PCG64 seeded with `seed`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n):
+        out.append(dict(npx=int(rng.choice(np.arange(32, 97, 8))), npy=int(rng.choice(np.arange(32, 97, 8))),
+                        npz=int(rng.integers(32, 65)), ra=float(10 ** rng.uniform(4, 6)),
+                        ta=float(10 ** rng.uniform(2, 6)), contrast=float(rng.uniform(3, 30)), seed=seed + 1 + b))
+    return out
+
+
+def ensemble_box(spec: dict):
+    """(meta, fields) of one ensemble box: Pr=1, Pm=1, vertical rotation
+    axis, tanh z grid, T perturbation 1e-4, seed field |k| <= 4 at 1e-5."""
+    return convection_box(spec["npx"], spec["npy"], spec["npz"], seed=spec["seed"], contrast=spec["contrast"],
+                          ra=spec["ra"], pr=1.0, pm=1.0, ta=spec["ta"], tilt_deg=0.0, ampt=1e-4, kmax=4,
+                          ampb=1e-5, ngrid=3)
+
+
 def divergence_b(meta: dict, st: dict) -> np.ndarray:
     """Centred-difference div B on the interior planes 2..npz-1 (padded
     arrays with periodic x/y ghosts), with the grid metrics of the step."""

@@ -2,9 +2,9 @@
 FP64 accumulation) against the FP64 form and the oracle.
 
 The mixed form has no counterpart in the reference; its check is the
-error bound: after 1000 steps each stored field must agree with the FP64
-result (bit-identical to the reference) to 1e-6 relative L2, and the state
-must match the oracle to the same bound.  Deterministic behaviour
+error bound: after 1000 steps at the same fixed dt each stored field must
+agree with the FP64 result (bit-identical to the reference) to 1e-6
+relative L2, and the state must match the oracle to the same bound.  Deterministic behaviour
 (batching, periodic shifts) is checked bit for bit.
 """
 
@@ -29,11 +29,19 @@ def wl():
     return importlib.import_module("rast-3dmhd_amd.workload")
 
 
-def run(meta, st, precision, n):
+def run(meta, st, precision, n, fixed_dt=0.0):
     with H.pkg().Box(meta, precision=precision) as box:
         box.upload(st)
+        if fixed_dt:
+            box.set_fixed_dt(fixed_dt)
         dt, um = box.steps(n)
         return box.download(), dt, um, box.clock()
+
+
+def first_dt(meta, st):
+    with H.pkg().Box(meta) as box:
+        box.upload(st)
+        return box.step()[0]
 
 
 def test_mixed_upload_download_round_trip():
@@ -50,13 +58,14 @@ def test_mixed_upload_download_round_trip():
 
 @pytest.mark.parametrize("lmag,lrot", [(True, True), (False, False)])
 def test_mixed_vs_fp64_1000_steps(lmag, lrot):
+    """Both forms at the same fixed dt (equal simulated times)."""
     say("case mixed vs fp64 1000 steps", lmag, lrot)
     meta, st = wl().convection_box(96, 64, 48, seed=5, lmag=lmag, lrot=lrot)
-    ref, dt64, um64, c64 = run(meta, st, "fp64", 1000)
-    mix, dtm, umm, cm = run(meta, st, "mixed", 1000)
-    assert c64[0] == cm[0] == 1000
-    assert abs(cm[1] - c64[1]) <= 1e-6 * c64[1], (cm[1], c64[1])
-    assert abs(dtm - dt64) <= 1e-6 * dt64
+    dt0 = first_dt(meta, st)
+    ref, dt64, um64, c64 = run(meta, st, "fp64", 1000, fixed_dt=dt0)
+    mix, dtm, umm, cm = run(meta, st, "mixed", 1000, fixed_dt=dt0)
+    assert c64 == cm and c64[0] == 1000 and dt64 == dtm == dt0
+    assert abs(umm - um64) <= 1e-6 * um64
     err = wl().field_errors(meta, mix, ref)
     say("rel_l2", err["rel_l2"], "velocity", err["velocity"], "entropy", err["entropy"])
     for k, e in err["rel_l2"].items():
@@ -65,6 +74,34 @@ def test_mixed_vs_fp64_1000_steps(lmag, lrot):
     if lmag:
         d = err["divb_max"]
         assert abs(d["test"] - d["ref"]) <= 1e-3 * d["ref"] + 1e-12 * d["b_max"], d
+
+
+def test_mixed_vs_fp64_adaptive_dt():
+    """With each form's own CFL dt the simulated times differ slightly; the
+    fields still agree closely and the times to 1e-6."""
+    say("case mixed vs fp64 adaptive")
+    meta, st = wl().convection_box(96, 64, 48, seed=5)
+    ref, dt64, _, c64 = run(meta, st, "fp64", 1000)
+    mix, dtm, _, cm = run(meta, st, "mixed", 1000)
+    assert cm[0] == c64[0] == 1000
+    assert abs(cm[1] - c64[1]) <= 1e-6 * c64[1], (cm[1], c64[1])
+    err = wl().field_errors(meta, mix, ref)
+    say("rel_l2", err["rel_l2"])
+    assert err["max_rel_l2"] <= 1e-5
+
+
+def test_fixed_dt_equal_to_cfl_dt_is_bitwise_adaptive():
+    """Fixing dt to the value the CFL reduction gives reproduces the adaptive
+    step bit for bit (FP64 form), and the clock advances by that dt."""
+    say("case fixed dt")
+    meta, st = wl().convection_box(64, 40, 24, seed=2)
+    a, dta, _, ca = run(meta, st, "fp64", 1)
+    b, dtb, _, cb = run(meta, st, "fp64", 1, fixed_dt=dta)
+    assert dta == dtb and ca == cb
+    for k in H.FIELDS:
+        assert np.array_equal(a[k], b[k]), k
+    _, dt3, _, c3 = run(meta, st, "fp64", 3, fixed_dt=0.5 * dta)
+    assert dt3 == 0.5 * dta and c3[0] == 3 and c3[1] == (0.5 * dta + 0.5 * dta) + 0.5 * dta
 
 
 def test_mixed_vs_oracle():

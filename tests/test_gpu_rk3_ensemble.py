@@ -1,0 +1,135 @@
+"""Batched ensemble (persistent launches over all boxes, HIP-graph replay,
+autotuned tiling) against the single-box path.
+
+Every box of an ensemble must advance bit for bit as the single-box
+library advances it alone (FP64: which is itself bit-identical to the
+reference; mixed: the same mixed kernels), with its own DT and clock,
+whatever the tiling, with or without graph replay, and after an autotune.
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+# uneven boxes: partial 64-wide tiles, odd heights, tiny and several chunks
+SPECS = [
+    dict(npx=40, npy=32, npz=33, ra=2e4, ta=1e3, contrast=5.0, seed=101),
+    dict(npx=96, npy=56, npz=40, ra=3e5, ta=1e5, contrast=20.0, seed=102),
+    dict(npx=72, npy=88, npz=64, ra=1e6, ta=1e6, contrast=30.0, seed=103),
+    dict(npx=32, npy=96, npz=47, ra=1e4, ta=1e2, contrast=3.0, seed=104),
+    dict(npx=64, npy=40, npz=32, ra=6e4, ta=4e4, contrast=10.0, seed=105),
+]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def boxes(specs=SPECS):
+    wl = importlib.import_module("rast-3dmhd_amd.workload")
+    return [wl.ensemble_box(s) for s in specs]
+
+
+def alone(meta, st, precision, n):
+    with H.pkg().Box(meta, precision=precision) as box:
+        box.upload(st)
+        dt, um = box.steps(n)
+        return box.download(), dt, um, box.clock()
+
+
+def check_equal(ens, refs):
+    for b, (st_ref, dt, um, clk) in enumerate(refs):
+        got = ens.download(b)
+        assert ens.clock(b) == clk, (b, ens.clock(b), clk)
+        for k in H.FIELDS:
+            assert np.array_equal(got[k], st_ref[k]), (b, k, np.abs(got[k] - st_ref[k]).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ensemble_bitwise_equal_single_box(precision):
+    say("case ensemble vs single", precision)
+    bx = boxes()
+    nsteps = 7  # one direct step, pair replays, a single-step replay
+    refs = [alone(m, s, precision, nsteps) for m, s in bx]
+    pkg = H.pkg()
+    with pkg.Ensemble([m for m, _ in bx], precision=precision) as ens:
+        for b, (_, s) in enumerate(bx):
+            ens.upload(b, s)
+        dt, um = ens.steps(nsteps)
+        for b, r in enumerate(refs):
+            assert dt[b] == r[1] and um[b] == r[2], (b, dt[b], r[1])
+        check_equal(ens, refs)
+
+
+@pytest.mark.parametrize("tiling", [(4, 8, 0), (8, 0, 256), (4, 16, 512), (8, 32, 1024)])
+def test_ensemble_tiling_invariance(tiling):
+    say("case ensemble tiling", tiling)
+    bx = boxes(SPECS[:3])
+    refs = [alone(m, s, "mixed", 4) for m, s in bx]
+    with H.pkg().Ensemble([m for m, _ in bx], precision="mixed") as ens:
+        ens.set_tiling(*tiling)
+        assert ens.tiling() == tiling
+        for b, (_, s) in enumerate(bx):
+            ens.upload(b, s)
+        ens.steps(2, graph=False)
+        ens.steps(2, graph=True)
+        check_equal(ens, refs)
+
+
+def test_ensemble_autotune_keeps_state():
+    say("case ensemble autotune")
+    bx = boxes()
+    refs = [alone(m, s, "fp64", 5) for m, s in bx]
+    with H.pkg().Ensemble([m for m, _ in bx], precision="fp64") as ens:
+        for b, (_, s) in enumerate(bx):
+            ens.upload(b, s)
+        ens.steps(2)
+        report = ens.autotune(nsteps=2)
+        say("autotune", ens.tiling(), min(r["ms_step"] for r in report))
+        assert len(report) == 32
+        best = min(report, key=lambda r: r["ms_step"])
+        assert ens.tiling() == (best["tile_y"], best["kdepth"], best["grid"])
+        ens.steps(3)
+        check_equal(ens, refs)
+
+
+def test_ensemble_reupload_one_box():
+    """Uploading one box again restarts that box only."""
+    say("case ensemble re-upload")
+    bx = boxes(SPECS[:2])
+    with H.pkg().Ensemble([m for m, _ in bx], precision="fp64") as ens:
+        for b, (_, s) in enumerate(bx):
+            ens.upload(b, s)
+        ens.steps(3)
+        ens.upload(1, bx[1][1])
+        ens.set_clock(1, 0, 0.0, 0.0)
+        ens.steps(2)
+        r0 = alone(*bx[0], "fp64", 5)
+        r1 = alone(*bx[1], "fp64", 2)
+        check_equal(ens, [r0, r1])
+
+
+def test_ensemble_errors():
+    say("case ensemble errors")
+    pkg = H.pkg()
+    bx = boxes(SPECS[:2])
+    with pkg.Ensemble([m for m, _ in bx], precision="fp64") as ens:
+        with pytest.raises(pkg.MhdError) as e:
+            ens.steps(1)
+        assert e.value.code == pkg.MHD_ERR_STATE
+        with pytest.raises(pkg.MhdError) as e:
+            ens.set_tiling(5, 8, 0)
+        assert e.value.code == pkg.MHD_ERR_ARG
+        with pytest.raises(pkg.MhdError) as e:
+            ens.download(7)
+        assert e.value.code == pkg.MHD_ERR_ARG
+    m0 = dict(bx[0][0])
+    m0["lmag"] = 0
+    with pytest.raises(pkg.MhdError) as e:
+        pkg.Ensemble([m0, bx[1][0]])
+    assert e.value.code == pkg.MHD_ERR_ARG

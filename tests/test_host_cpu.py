@@ -13,11 +13,16 @@ import pytest
 import helpers as H
 
 HEADER = os.path.join(H.REPO, "include", "rast3dmhd.h")
+HEADERS = sorted(os.path.join(H.REPO, "include", f) for f in os.listdir(os.path.join(H.REPO, "include"))
+                 if f.endswith(".h"))
 
 
 def declared_symbols():
-    txt = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(mhd_[a-z_]+)\s*\(", txt)))
+    names = set()
+    for h in HEADERS:
+        txt = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(mhd_[a-z_0-9]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -25,9 +30,16 @@ def test_library_exports_every_declared_symbol():
     assert os.path.exists(pkg.LIB_PATH), "build the HIP library first (__graft_entry__.build())"
     lib = ctypes.CDLL(pkg.LIB_PATH)
     names = declared_symbols()
-    assert len(names) >= 11
+    assert len(names) >= 28
     for n in names:
         assert hasattr(lib, n), n
+
+
+def test_python_binding_declares_every_symbol():
+    """load() sets argtypes for every entry point of include/*.h."""
+    src = open(os.path.join(H.REPO, "rast-3dmhd_amd", "__init__.py")).read()
+    for n in declared_symbols():
+        assert f"lib.{n}.argtypes" in src, n
 
 
 def test_desc_layout_matches_header():
