@@ -126,6 +126,17 @@ int mhd_set_fixed_dt(mhd_t *h, double dt);
 int mhd_fluxes(mhd_t *h, double *fu, double *fv, double *fw, double *fr, double *ft,
                double *ww1, double *ww2, double *ww3);
 
+/* No counterpart in the reference (training data of the subgrid
+ * closure): box filter of the state (source 0: RU..BZ) or of the
+ * right-hand side at the state (source 1: FU,FV,FW,FR,FT,WW1..WW3) onto a
+ * grid coarser by `factor`.  Horizontally the mean of factor x factor
+ * blocks; vertically the mean of factor planes (zmode 0, NPZ divisible by
+ * factor) or, for vertex grids with NPZ = factor*(nzc-1)+1 and even factor,
+ * trapezoid weights over the factor spacings centred on every factor-th
+ * plane, the two boundary planes taken as they are (zmode 1).  out is
+ * [field][K][J][I] with I fastest, interior points only.  MHD_FP64 only. */
+int mhd_box_filter(mhd_t *h, int factor, int zmode, int source, double *out);
+
 /* Measurement hook (no counterpart in the reference): runs n steps like
  * mhd_steps with HIP events around every launch on the handle's stream and
  * returns the mean duration in milliseconds of one step, one RK stage

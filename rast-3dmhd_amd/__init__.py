@@ -68,6 +68,7 @@ def load(path: str | None = None):
     lib.mhd_set_clock.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
     lib.mhd_fluxes.argtypes = [C.c_void_p] + [_P] * 8
     lib.mhd_set_fixed_dt.argtypes = [C.c_void_p, C.c_double]
+    lib.mhd_box_filter.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _P]
     lib.mhd_time_steps.argtypes = [C.c_void_p, C.c_int, _P, _P, _P, _P]
     lib.mhd_last_error.argtypes = [C.c_void_p]
     lib.mhd_last_error.restype = C.c_char_p
@@ -93,12 +94,34 @@ def load(path: str | None = None):
     lib.mhd_ens_last_error.restype = C.c_char_p
     lib.mhd_ens_destroy.argtypes = [V]
     lib.mhd_ens_destroy.restype = None
+    F = C.POINTER(C.c_float)
+    lib.mhd_net_create.argtypes = [I, I, I, C.POINTER(V)]
+    lib.mhd_net_num_params.argtypes = [V]
+    lib.mhd_net_num_params.restype = C.c_longlong
+    lib.mhd_net_set_params.argtypes = [V, F]
+    lib.mhd_net_get_params.argtypes = [V, F]
+    lib.mhd_net_forward.argtypes = [V, I, I, I, F, F]
+    lib.mhd_net_gradient.argtypes = [V, I, I, I, F, F, F, F]
+    lib.mhd_net_add_sample.argtypes = [V, I, I, I, F, F]
+    lib.mhd_net_num_samples.argtypes = [V]
+    lib.mhd_net_clear_samples.argtypes = [V]
+    lib.mhd_net_train.argtypes = [V, I, C.c_float, _P, _P]
+    lib.mhd_net_loss.argtypes = [V, I, _P]
+    lib.mhd_net_time_forward.argtypes = [V, I, I, I, I, _P]
+    lib.mhd_net_last_error.argtypes = [V]
+    lib.mhd_net_last_error.restype = C.c_char_p
+    lib.mhd_net_destroy.argtypes = [V]
+    lib.mhd_net_destroy.restype = None
+    for fn in ("mhd_net_create", "mhd_net_set_params", "mhd_net_get_params", "mhd_net_forward", "mhd_net_gradient",
+               "mhd_net_add_sample", "mhd_net_num_samples", "mhd_net_clear_samples", "mhd_net_train",
+               "mhd_net_loss", "mhd_net_time_forward"):
+        getattr(lib, fn).restype = C.c_int
     for fn in ("mhd_ens_create", "mhd_ens_set_metrics", "mhd_ens_set_background", "mhd_ens_upload",
                "mhd_ens_download", "mhd_ens_set_clock", "mhd_ens_get_clock", "mhd_ens_set_fixed_dt", "mhd_ens_steps", "mhd_ens_set_tiling",
                "mhd_ens_get_tiling", "mhd_ens_autotune", "mhd_ens_time_steps"):
         getattr(lib, fn).restype = C.c_int
     for fn in ("mhd_create", "mhd_set_metrics", "mhd_set_background", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
-               "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps", "mhd_set_fixed_dt"):
+               "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps", "mhd_set_fixed_dt", "mhd_box_filter"):
         getattr(lib, fn).restype = C.c_int
     _lib = lib
     return lib
@@ -209,6 +232,16 @@ class Box:
     def set_fixed_dt(self, dt: float):
         """dt > 0: step with this DT from now on; 0: back to the CFL DT."""
         self._check(self.lib.mhd_set_fixed_dt(self.h, float(dt)))
+
+    def box_filter(self, factor: int, zmode: int = 1, source: str = "state") -> np.ndarray:
+        """Box-filtered state ("state") or right-hand side ("rhs") on the grid
+        coarser by `factor`: array [field][K][J][I]."""
+        d = self.desc
+        nzc = d.npz // factor if zmode == 0 else (d.npz - 1) // factor + 1
+        nf = 8 if self.lmag else 5
+        out = np.zeros((nf, nzc, d.npy // factor, d.npx // factor))
+        self._check(self.lib.mhd_box_filter(self.h, int(factor), int(zmode), 0 if source == "state" else 1, _ptr(out)))
+        return out
 
     def fluxes(self) -> dict:
         out = {k: np.zeros(self.shape, order="F") for k in RHS}
@@ -343,6 +376,104 @@ class Ensemble:
     def close(self):
         if getattr(self, "h", None):
             self.lib.mhd_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _fptr(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Net:
+    """The closure network (include/rast3dmhd_closure.h) on the current GPU:
+    3x3x3 convolutions cin -> width -> width -> cout, GELU, BF16 MFMA."""
+
+    def __init__(self, cin: int = 8, width: int = 32, cout: int = 7):
+        self.lib = load()
+        self.cin, self.width, self.cout = cin, width, cout
+        h = C.c_void_p()
+        rc = self.lib.mhd_net_create(cin, width, cout, C.byref(h))
+        if rc:
+            raise MhdError(rc, self.lib.mhd_net_last_error(None).decode())
+        self.h = h
+        self.nparam = int(self.lib.mhd_net_num_params(self.h))
+
+    def _check(self, rc):
+        if rc:
+            raise MhdError(rc, self.lib.mhd_net_last_error(self.h).decode())
+
+    def set_params(self, vec):
+        v = np.ascontiguousarray(vec, dtype=np.float32)
+        if v.size != self.nparam:
+            raise ValueError(f"{v.size} parameters, expected {self.nparam}")
+        self._check(self.lib.mhd_net_set_params(self.h, _fptr(v)))
+
+    def params(self):
+        v = np.zeros(self.nparam, dtype=np.float32)
+        self._check(self.lib.mhd_net_get_params(self.h, _fptr(v)))
+        return v
+
+    @staticmethod
+    def _vol(x, c):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 4 or x.shape[3] != c:
+            raise ValueError(f"expected [nz][ny][nx][{c}], got {x.shape}")
+        return x
+
+    def forward(self, x):
+        x = self._vol(x, self.cin)
+        nz, ny, nx, _ = x.shape
+        y = np.zeros((nz, ny, nx, self.cout), dtype=np.float32)
+        self._check(self.lib.mhd_net_forward(self.h, nx, ny, nz, _fptr(x), _fptr(y)))
+        return y
+
+    def gradient(self, x, dy, want_dx=False):
+        x = self._vol(x, self.cin)
+        dy = self._vol(dy, self.cout)
+        nz, ny, nx, _ = x.shape
+        g = np.zeros(self.nparam, dtype=np.float32)
+        dx = np.zeros_like(x) if want_dx else None
+        self._check(self.lib.mhd_net_gradient(self.h, nx, ny, nz, _fptr(x), _fptr(dy), _fptr(g), _fptr(dx)))
+        return (g, dx) if want_dx else g
+
+    def add_sample(self, x, t):
+        x = self._vol(x, self.cin)
+        t = self._vol(t, self.cout)
+        nz, ny, nx, _ = x.shape
+        self._check(self.lib.mhd_net_add_sample(self.h, nx, ny, nz, _fptr(x), _fptr(t)))
+
+    def clear_samples(self):
+        self._check(self.lib.mhd_net_clear_samples(self.h))
+
+    def train(self, iters: int, lr: float):
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.mhd_net_train(self.h, int(iters), float(lr), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def loss(self, s: int):
+        v = C.c_double()
+        self._check(self.lib.mhd_net_loss(self.h, int(s), C.byref(v)))
+        return v.value
+
+    def time_forward(self, nx, ny, nz, reps=10):
+        v = C.c_double()
+        self._check(self.lib.mhd_net_time_forward(self.h, nx, ny, nz, reps, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mhd_net_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -414,6 +414,38 @@ extern "C" int mhd_fluxes(mhd_t *h, double *fu, double *fv, double *fw, double *
     return MHD_OK;
 }
 
+extern "C" int mhd_box_filter(mhd_t *h, int factor, int zmode, int source, double *out) {
+    int rc = ready(h, "mhd_box_filter");
+    if (rc) return rc;
+    if (h->mixed) return fail(h, MHD_ERR_SCOPE, "mhd_box_filter: only for MHD_FP64 handles");
+    if (!out) return fail(h, MHD_ERR_ARG, "mhd_box_filter: null output");
+    const Params &p = h->p;
+    if (factor < 1 || p.npx % factor || p.npy % factor || (zmode != 0 && zmode != 1) || (source != 0 && source != 1))
+        return fail(h, MHD_ERR_ARG, "mhd_box_filter: factor %d, zmode %d, source %d for %dx%dx%d", factor, zmode,
+                    source, p.npx, p.npy, p.npz);
+    if (zmode == 0 ? p.npz % factor != 0 : ((p.npz - 1) % factor != 0 || factor % 2 != 0))
+        return fail(h, MHD_ERR_ARG, "mhd_box_filter: npz=%d does not coarsen by %d (zmode %d)", p.npz, factor, zmode);
+    const int nxc = p.npx / factor, nyc = p.npy / factor;
+    const int nzc = zmode == 0 ? p.npz / factor : (p.npz - 1) / factor + 1;
+    const int nf = nfields(h);
+    HIPCHK(h, hipSetDevice(h->dev));
+    const double *src = h->q[h->cur];
+    if (source == 1) {
+        if (!h->f) HIPCHK(h, hipMalloc(&h->f, (size_t)p.fsize * sizeof(double) * nf));
+        HIPCHK(h, launch_stage(0, p, h->m, h->q[h->cur], nullptr, nullptr, h->f, h->sc, h->stream));
+        src = h->f;
+    }
+    const size_t bytes = (size_t)nf * nzc * nyc * nxc * sizeof(double);
+    double *d = nullptr;
+    HIPCHK(h, hipMalloc(&d, bytes));
+    hipError_t e = launch_box_filter(p, src, nf, factor, zmode, nxc, nyc, nzc, d, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(h, MHD_ERR_HIP, "mhd_box_filter: %s", hipGetErrorString(e));
+    return MHD_OK;
+}
+
 extern "C" const char *mhd_last_error(const mhd_t *h) { return h ? h->err : g_create_err; }
 
 extern "C" void mhd_destroy(mhd_t *h) {

@@ -112,6 +112,8 @@ hipError_t launch_bcon_batch(int lmag, const BoxDev *boxes, const ColumnItem *it
                              hipStream_t s);
 hipError_t launch_cfl_final_batch(const BoxDev *boxes, int nbox, bool clock, hipStream_t s);
 int stage_lds(int tile_y, int lmag, int mixed);
+hipError_t launch_box_filter(const Params &p, const double *q, int nf, int factor, int zmode, int nxc, int nyc, int nzc,
+                             double *out, hipStream_t s);
 
 // Mixed-precision form (mhd_mixed.hip).
 hipError_t launch_stage_mixed(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,

@@ -660,6 +660,52 @@ __global__ void k_unpack(const Params p, const double *__restrict__ field, doubl
     }
 }
 
+// Box filter of nf resident fields (planes 1..npz) onto a coarse grid:
+// out[f][K][J][I], I fastest.  Horizontally the mean of factor x factor
+// blocks (coarse I covers fine i = factor*I .. factor*I + factor-1).
+// Vertically, zmode 0: the mean of factor planes (coarse K covers fine
+// planes factor*K+1 .. factor*K+factor); zmode 1 (vertex grids with npz =
+// factor*(nzc-1)+1): coarse K sits on fine plane factor*K+1 and averages the
+// planes within factor/2 of it with trapezoid weights (1/2 at both ends);
+// the two boundary planes are taken as they are.
+__global__ void k_box_filter(const Params p, const double *__restrict__ q, int nf, int factor, int zmode, int nxc,
+                             int nyc, int nzc, double *__restrict__ out) {
+    const long long n = (long long)nf * nzc * nyc * nxc;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int I = (int)(e % nxc);
+        long long r = e / nxc;
+        const int J = (int)(r % nyc);
+        r /= nyc;
+        const int K = (int)(r % nzc);
+        const int f = (int)(r / nzc);
+        const double *qf = q + (long long)f * p.fsize;
+        auto hmean = [&](int k) {
+            double s = 0.0;
+            for (int jj = 0; jj < factor; ++jj) {
+                const double *row = qf + (long long)k * p.plane + (long long)(J * factor + jj) * p.npx + I * factor;
+                for (int ii = 0; ii < factor; ++ii) s += row[ii];
+            }
+            return s / (double)(factor * factor);
+        };
+        double v = 0.0;
+        if (zmode == 0) {
+            for (int kk = 1; kk <= factor; ++kk) v += hmean(K * factor + kk);
+            v /= (double)factor;
+        } else if (K == 0 || K == nzc - 1) {
+            v = hmean(K * factor + 1);
+        } else {
+            const int c = K * factor + 1, hw = factor / 2;
+            for (int d = -hw; d <= hw; ++d) {
+                const double w = (d == -hw || d == hw) ? 0.5 : 1.0;
+                v += w * hmean(c + d);
+            }
+            v /= (double)factor;
+        }
+        out[e] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 
@@ -818,6 +864,15 @@ hipError_t launch_unpack(const Params &p, const double *field, double *padded, i
     long long nb = (n + 255) / 256;
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(k_unpack, dim3((unsigned)nb), dim3(256), 0, s, p, field, padded, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_box_filter(const Params &p, const double *q, int nf, int factor, int zmode, int nxc, int nyc, int nzc,
+                             double *out, hipStream_t s) {
+    const long long n = (long long)nf * nzc * nyc * nxc;
+    long long nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_box_filter, dim3((unsigned)nb), dim3(256), 0, s, p, q, nf, factor, zmode, nxc, nyc, nzc, out);
     return hipGetLastError();
 }
 

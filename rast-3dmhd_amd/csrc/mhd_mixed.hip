@@ -16,7 +16,7 @@
 //
 // The RK3 accumulation runs in FP64 in registers, on the same variables as
 // the FP64 form (T, not P): X = value + D + gamma dt F, where the value of
-// the temperature slot is T'(rho', P') evaluated in FP64.  The second
+// the temperature slot is T'(rho', P') as the stencil computes it (trep).  The second
 // register D (FP32) carries, per Wray's low-storage scheme, the zeta*dt*F
 // increment of the next stage plus the rounding residual of the stored
 // value (for T: X_T - T'(new rho', new P')), and after the last stage the
@@ -53,14 +53,20 @@ constexpr int mixed_lds_bytes() {
     return (2 * (MAG ? 8 : 5) + 12) * MTile<TY>::SP * (int)sizeof(float);
 }
 
-// T' represented by a stored (rho', P') pair: (P' - T0 rho') / (rho0 + rho').
-__host__ __device__ __forceinline__ double tprime(double r0, double t0, float rh, float pp) {
-    return ((double)pp - t0 * (double)rh) / (r0 + (double)rh);
+// T' represented by a stored (rho', P') pair: (P' - T0 rho') / (rho0 + rho')
+// in FP32, with the operations of the stencil's ring (put_ring), so that the
+// value the residual D compensates against is bit-identical to the value
+// the right-hand side sees.
+__host__ __device__ __forceinline__ float trep(float r0, float t0, float rh, float pp) {
+    const float ri = 1.0f / (r0 + rh);
+    return (pp - t0 * rh) * ri;
 }
 
-// P' of temperature perturbation tp at stored density perturbation rh.
-__host__ __device__ __forceinline__ double pprime(double r0, double t0, float rh, double tp) {
-    return r0 * tp + t0 * (double)rh + (double)rh * tp;
+// P' of temperature perturbation tp at density perturbation rh.  The
+// stored P' is formed from the exact (FP64) rho', not from its FP32
+// rounding: T0 times the rounding error of rho' would be comparable to P'.
+__host__ __device__ __forceinline__ double pprime(double r0, double t0, double rh, double tp) {
+    return r0 * tp + t0 * rh + rh * tp;
 }
 
 struct RawF {
@@ -119,7 +125,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         ring(k, 0, pt) = r.v[0] * ri;
         ring(k, 1, pt) = r.v[1] * ri;
         ring(k, 2, pt) = r.v[2] * ri;
-        ring(k, 3, pt) = (r.v[4] - bg.tt32[k] * r.v[3]) * ri;  // temperature perturbation
+        ring(k, 3, pt) = (r.v[4] - bg.tt32[k] * r.v[3]) * ri;  // trep(): temperature perturbation
     };
 
     RawF rm_, rp_o, rp_h;
@@ -338,14 +344,14 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             float hn[8];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                const double v = f == 4 ? tprime(r0d, t0d, h0[3], h0[4]) : (double)h0[f];
+                const double v = f == 4 ? (double)th0 : (double)h0[f];
                 xq[f] = (v + (double)dk_[f]) + coefg * Fd[f];
                 if (f != 4) hn[f] = (float)xq[f];
             }
-            hn[4] = (float)pprime(r0d, t0d, hn[3], xq[4]);
+            hn[4] = (float)pprime(r0d, t0d, xq[3], xq[4]);
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                const double r = xq[f] - (f == 4 ? tprime(r0d, t0d, hn[3], hn[4]) : (double)hn[f]);
+                const double r = xq[f] - (f == 4 ? (double)trep(bg.ro32[k], bg.tt32[k], hn[3], hn[4]) : (double)hn[f]);
                 __builtin_nontemporal_store(hn[f], &hout[f * fs + o]);
                 const float dn = (MODE < 3) ? (float)(r + coefz * Fd[f]) : (float)r;
                 __builtin_nontemporal_store(dn, &dr[f * fs + o]);
@@ -421,7 +427,7 @@ __device__ __forceinline__ void bcon_column_m(const Params &p, const Metrics &m,
         const double C13 = p.c13, C23 = p.c23, C43 = p.c43;
         auto full = [&](int f, int k) -> double {
             const long long o = f * fs + (long long)k * pl + idx;
-            double v = f == 4 ? tprime(bg.ro[k], bg.tt[k], h[o - fs], h[o]) : (double)h[o];
+            double v = f == 4 ? (double)trep(bg.ro32[k], bg.tt32[k], h[o - fs], h[o]) : (double)h[o];
             if (LAST) v += (double)dr[o];
             if (f == 3) v += bg.ro[k];
             if (f == 4) v += bg.tt[k];
@@ -432,9 +438,10 @@ __device__ __forceinline__ void bcon_column_m(const Params &p, const Metrics &m,
             if (f == 4) {
                 const double tp = v - bg.tt[k];
                 const float rh = h[o - fs];
-                const float hp = (float)pprime(bg.ro[k], bg.tt[k], rh, tp);
+                const double rhx = LAST ? (double)rh + (double)dr[o - fs] : (double)rh;
+                const float hp = (float)pprime(bg.ro[k], bg.tt[k], rhx, tp);
                 h[o] = hp;
-                dr[o] = (float)(tp - tprime(bg.ro[k], bg.tt[k], rh, hp));
+                dr[o] = (float)(tp - (double)trep(bg.ro32[k], bg.tt32[k], rh, hp));
                 return;
             }
             if (f == 3) v -= bg.ro[k];
@@ -506,7 +513,7 @@ __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, 
         const int j = (int)(r2 / p.npx), i = (int)(r2 - (long long)j * p.npx);
         double v[8];
         for (int f = 0; f < (MAG ? 8 : 5); ++f) {
-            const double x = f == 4 ? tprime(bg.ro[k], bg.tt[k], h[3 * fs + o], h[4 * fs + o]) : (double)h[f * fs + o];
+            const double x = f == 4 ? (double)trep(bg.ro32[k], bg.tt32[k], h[3 * fs + o], h[4 * fs + o]) : (double)h[f * fs + o];
             v[f] = x + (double)dr[f * fs + o];
         }
         v[3] += bg.ro[k];
@@ -534,9 +541,9 @@ __global__ void k_pack_m(const Params p, const double *__restrict__ padded, int 
         if (field == 4) {
             const double tp = v - bg.tt[k];
             const float rh = hreg[3 * fs + idx];
-            const float hp = (float)pprime(bg.ro[k], bg.tt[k], rh, tp);
+            const float hp = (float)pprime(bg.ro[k], bg.tt[k], (double)rh + (double)dreg[3 * fs + idx], tp);
             h[idx] = hp;
-            dr[idx] = (float)(tp - tprime(bg.ro[k], bg.tt[k], rh, hp));
+            dr[idx] = (float)(tp - (double)trep(bg.ro32[k], bg.tt32[k], rh, hp));
             continue;
         }
         if (field == 3) v -= bg.ro[k];
@@ -558,7 +565,7 @@ __global__ void k_unpack_m(const Params p, const float *__restrict__ hreg, const
         const int J = (int)(r / nxp), I = (int)(r - (long long)J * nxp);
         const long long o = k * p.plane + (long long)wrapi(J - 1, p.npy) * p.npx + wrapi(I - 1, p.npx);
         double v;
-        if (field == 4) v = bg.tt[k] + tprime(bg.ro[k], bg.tt[k], hreg[3 * fs + o], hreg[4 * fs + o]);
+        if (field == 4) v = bg.tt[k] + (double)trep(bg.ro32[k], bg.tt32[k], hreg[3 * fs + o], hreg[4 * fs + o]);
         else v = (double)hreg[field * fs + o] + (field == 3 ? bg.ro[k] : 0.0);
         padded[idx] = v + (double)dreg[field * fs + o];
     }

@@ -125,9 +125,9 @@ def test_ensemble_errors():
         with pytest.raises(pkg.MhdError) as e:
             ens.set_tiling(5, 8, 0)
         assert e.value.code == pkg.MHD_ERR_ARG
-        with pytest.raises(pkg.MhdError) as e:
-            ens.download(7)
-        assert e.value.code == pkg.MHD_ERR_ARG
+        # the C entry point checks the box index itself
+        assert ens.lib.mhd_ens_download(ens.h, 7, *([None] * 8)) == pkg.MHD_ERR_ARG
+        assert ens.lib.mhd_ens_download(ens.h, -1, *([None] * 8)) == pkg.MHD_ERR_ARG
     m0 = dict(bx[0][0])
     m0["lmag"] = 0
     with pytest.raises(pkg.MhdError) as e:

@@ -1,0 +1,101 @@
+"""The closure network's BF16 MFMA kernels against the CPU oracle
+(oracle/net_oracle.py, pinned by finite differences): forward pass,
+parameter and input gradients, and training on the device."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+sys.path.insert(0, os.path.join(H.REPO, "oracle"))
+import net_oracle as N  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+CIN, WIDTH, COUT = 8, 32, 7
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def rel(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
+
+
+@pytest.mark.parametrize("shape", [(6, 8, 32), (5, 9, 70), (3, 4, 16)])
+def test_forward_matches_oracle(shape):
+    nz, ny, nx = shape
+    say("case net forward", shape)
+    rng = np.random.default_rng(sum(shape))
+    p = N.init_params(CIN, WIDTH, COUT, seed=1)
+    x = N.bf16(rng.standard_normal((nz, ny, nx, CIN)))
+    ref = N.Net(p, bf16_emulation=True).forward(x)
+    with H.pkg().Net(CIN, WIDTH, COUT) as net:
+        net.set_params(N.flatten(p))
+        y = net.forward(x)
+    say("rel", rel(y, ref), "max", np.abs(y - ref).max() / np.abs(ref).max())
+    assert rel(y, ref) <= 2e-3
+    assert np.abs(y - ref).max() <= 2e-2 * np.abs(ref).max()
+
+
+def test_gradients_match_oracle():
+    say("case net gradient")
+    nz, ny, nx = 4, 8, 32
+    rng = np.random.default_rng(7)
+    p = N.init_params(CIN, WIDTH, COUT, seed=2)
+    x = N.bf16(rng.standard_normal((nz, ny, nx, CIN)))
+    dy = N.bf16(rng.standard_normal((nz, ny, nx, COUT)))
+    g_ref, dx_ref = N.Net(p, bf16_emulation=True).backward(x, dy)
+    with H.pkg().Net(CIN, WIDTH, COUT) as net:
+        net.set_params(N.flatten(p))
+        g, dx = net.gradient(x, dy, want_dx=True)
+    got = N.unflatten(g, CIN, WIDTH, COUT)
+    for k in ("W3", "b3", "W2", "b2", "W1", "b1"):
+        e = rel(got[k], g_ref[k])
+        say(k, e)
+        assert e <= 1e-2, (k, e)
+    assert rel(dx, dx_ref) <= 1e-2
+
+
+def test_training_reduces_loss():
+    """Fit the outputs of a teacher network with other weights."""
+    say("case net training")
+    nz, ny, nx = 8, 16, 32
+    rng = np.random.default_rng(11)
+    teacher = N.init_params(CIN, WIDTH, COUT, seed=3)
+    with H.pkg().Net(CIN, WIDTH, COUT) as net:
+        net.set_params(N.flatten(teacher))
+        xs = [N.bf16(rng.standard_normal((nz, ny, nx, CIN))) for _ in range(2)]
+        ts = [net.forward(x) for x in xs]
+        net.set_params(N.flatten(N.init_params(CIN, WIDTH, COUT, seed=4)))
+        for x, t in zip(xs, ts):
+            net.add_sample(x, t)
+        l0 = net.loss(0)
+        first, last = net.train(1200, 3e-3)
+        l1 = net.loss(0)
+        say("loss", l0, first, last, l1)
+    assert first == pytest.approx(l0, rel=1e-6)
+    assert l1 < 0.1 * l0
+
+
+@pytest.mark.parametrize("npz,zmode", [(17, 1), (16, 0)])
+def test_box_filter_matches_oracle(npz, zmode):
+    say("case box filter", npz, zmode)
+    from test_gpu_parity import random_box
+    meta, st = random_box(64, 32, npz, True, True, seed=npz)
+    with H.pkg().Box(meta) as box:
+        box.upload(st)
+        got = box.box_filter(4, zmode=zmode, source="state")
+        got_rhs = box.box_filter(4, zmode=zmode, source="rhs")
+        rhs = box.fluxes()
+    filt = N.box_filter_vertex if zmode == 1 else N.box_filter
+    for f, k in enumerate(H.FIELDS):
+        ref = filt(st[k][1:-1, 1:-1, 1:-1].transpose(2, 1, 0), 4)
+        assert np.allclose(got[f], ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max()), k
+    for f, k in enumerate(["fu", "fv", "fw", "fr", "ft", "ww1", "ww2", "ww3"]):
+        ref = filt(rhs[k][1:-1, 1:-1, 1:-1].transpose(2, 1, 0), 4)
+        assert np.allclose(got_rhs[f], ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max()), k
