@@ -58,6 +58,22 @@ int mhd_net_loss(mhd_net_t *n, int s, double *loss);
  * (HIP events; reps passes after one warm-up). */
 int mhd_net_time_forward(mhd_net_t *n, int nx, int ny, int nz, int reps, double *ms);
 
+/* Closure-corrected step of a MHD_MIXED box (mhd_t of rast3dmhd.h): every
+ * following step evaluates the network on the state at its start and adds
+ * its output to the right-hand sides of RU, RV, RW, TT, BX, BY, BZ in all
+ * three stages.  Inputs (rho, u, v, w, s = ln T / (gamma - 1) - ln rho, BX,
+ * BY, BZ) enter as (value - in_profile[c][k]) / in_std[c]; the forcing is
+ * output * out_std[c] + out_profile[c][k], where the profiles hold one value
+ * per interior plane k = 0..NPZ-1 (in_profile [8][NPZ], out_profile [7][NPZ]).
+ * Fallback: with net NULL, or val_error (the network's error on held-out
+ * data) above threshold, the box steps plainly; a step whose network output
+ * is not finite is also plain.  The network must live on the box's device
+ * and stay alive while attached. */
+int mhd_set_closure(mhd_t *h, mhd_net_t *net, const double *in_profile, const double *in_std,
+                    const double *out_profile, const double *out_std, double val_error, double threshold);
+/* 1 if a network is attached (mhd_set_closure accepted it), else 0. */
+int mhd_closure_active(const mhd_t *h);
+
 const char *mhd_net_last_error(const mhd_net_t *n);
 void mhd_net_destroy(mhd_net_t *n);
 

@@ -108,6 +108,10 @@ def load(path: str | None = None):
     lib.mhd_net_train.argtypes = [V, I, C.c_float, _P, _P]
     lib.mhd_net_loss.argtypes = [V, I, _P]
     lib.mhd_net_time_forward.argtypes = [V, I, I, I, I, _P]
+    lib.mhd_set_closure.argtypes = [V, V, _P, _P, _P, _P, D, D]
+    lib.mhd_set_closure.restype = C.c_int
+    lib.mhd_closure_active.argtypes = [V]
+    lib.mhd_closure_active.restype = C.c_int
     lib.mhd_net_last_error.argtypes = [V]
     lib.mhd_net_last_error.restype = C.c_char_p
     lib.mhd_net_destroy.argtypes = [V]
@@ -232,6 +236,28 @@ class Box:
     def set_fixed_dt(self, dt: float):
         """dt > 0: step with this DT from now on; 0: back to the CFL DT."""
         self._check(self.lib.mhd_set_fixed_dt(self.h, float(dt)))
+
+    def set_closure(self, net, in_profile=None, in_std=None, out_profile=None, out_std=None, val_error: float = 0.0,
+                    threshold: float = 0.1):
+        """Attach a trained Net (mixed form).  Inputs enter as (value -
+        in_profile[c][k]) / in_std[c]; the forcing added to the right-hand
+        sides of RU, RV, RW, TT, BX, BY, BZ is output * out_std[c] +
+        out_profile[c][k] (profiles [8][npz] and [7][npz]).  net=None, or
+        val_error > threshold, leaves the box stepping plainly."""
+        nz = self.desc.npz
+        arrs = [None] * 4
+        if net is not None:
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (in_profile, in_std, out_profile, out_std)]
+            want = [(8, nz), (8,), (7, nz), (7,)]
+            for a, w in zip(arrs, want):
+                if a.shape != w:
+                    raise ValueError(f"closure normalisation shapes {[a.shape for a in arrs]}, expected {want}")
+        self._check(self.lib.mhd_set_closure(self.h, net.h if net is not None else None, *[_ptr(a) for a in arrs],
+                                             float(val_error), float(threshold)))
+        self._net = net  # keep it alive while attached
+
+    def closure_active(self) -> bool:
+        return bool(self.lib.mhd_closure_active(self.h))
 
     def box_filter(self, factor: int, zmode: int = 1, source: str = "state") -> np.ndarray:
         """Box-filtered state ("state") or right-hand side ("rhs") on the grid

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/rast3dmhd.h"
+#include "../../include/rast3dmhd_closure.h"
 #include "mhd_host.h"
 
 using namespace rmhd;
@@ -41,6 +42,14 @@ struct mhd_ctx {
     Metrics m{};
     std::vector<double> dzzdz, d2zzdz2, rkapa, dkapa;  // host copies for the background
     DevScalars *sc = nullptr;
+    // closure-corrected step (mixed form)
+    mhd_net_t *net = nullptr;
+    float *force = nullptr;    // [7][fsize]
+    int *force_on = nullptr;
+    __bf16 *cx = nullptr;      // network input [npz][npy][npx][8]
+    float *nrm = nullptr;      // input plane means [8][npz], 1/std[8]; then nrm_out:
+    float *nrm_out = nullptr;  // subgrid plane means [7][npz], std[7]
+    bool closure = false;
     int cur = 0;
     bool have_metrics = false, have_state = false, have_bg = false;
     bool cfl_valid = false;             // sc->dt/umach describe the current register
@@ -72,8 +81,8 @@ static void free_all(mhd_t *h) {
     if (!h) return;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *bufs[12] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->bgd, h->bgf,
-                      h->f, h->stage, h->met, h->sc};
+    void *bufs[16] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->bgd, h->bgf,
+                      h->f, h->stage, h->met, h->sc, h->force, h->force_on, h->cx, h->nrm};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -178,6 +187,10 @@ extern "C" int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0
     HIPCHK(h, hipMemcpyAsync(h->bgf, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->bg = background_view(h->p.npz, h->bgd, h->bgf);
+    if (h->closure) {
+        h->bg.force = h->force;
+        h->bg.force_on = h->force_on;
+    }
     h->have_bg = true;
     h->have_state = false;
     return MHD_OK;
@@ -246,6 +259,13 @@ static hipError_t launches(mhd_t *h, hipEvent_t *ev) {
         if (!h->mixed) e = launch_cfl(p, h->m, h->q[h->cur], h->sc, h->stream);
         else e = launch_cfl_mixed(p, h->m, h->bg, h->hq[h->cur], h->dr, h->sc, h->stream);
         if (e == hipSuccess) e = launch_cfl_final(p, h->sc, false, h->stream);
+    }
+    if (e == hipSuccess && h->closure) {
+        // subgrid forcing from the network on the state at the start of the step
+        const float *y = nullptr;
+        e = launch_closure_inputs(p, h->bg, h->hq[h->cur], h->dr, h->nrm, h->cx, h->stream);
+        if (e == hipSuccess) e = net_forward_device(h->net, h->stream, p.npx, p.npy, p.npz, h->cx, &y);
+        if (e == hipSuccess) e = launch_closure_forcing(p, y, h->nrm_out, h->force, h->force_on, h->stream);
     }
     rec(1);
     int a = h->cur, b = 1 - h->cur;
@@ -413,6 +433,59 @@ extern "C" int mhd_fluxes(mhd_t *h, double *fu, double *fv, double *fw, double *
     }
     return MHD_OK;
 }
+
+extern "C" int mhd_set_closure(mhd_t *h, mhd_net_t *net, const double *in_profile, const double *in_std,
+                               const double *out_profile, const double *out_std, double val_error,
+                               double threshold) {
+    if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_set_closure: null handle");
+    if (!h->mixed) return fail(h, MHD_ERR_SCOPE, "mhd_set_closure: only for MHD_MIXED handles");
+    HIPCHK(h, hipSetDevice(h->dev));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->closure = false;
+    h->net = nullptr;
+    h->bg.force = nullptr;
+    h->bg.force_on = nullptr;
+    // fallback: no network, or one whose validation error exceeds the threshold
+    if (!net || !(val_error <= threshold)) return MHD_OK;
+    if (!in_profile || !in_std || !out_profile || !out_std)
+        return fail(h, MHD_ERR_ARG, "mhd_set_closure: null normalisation");
+    int cin = 0, cout = 0;
+    net_shape(net, &cin, &cout);
+    if (cin != 8 || cout != 7)
+        return fail(h, MHD_ERR_ARG, "mhd_set_closure: network maps %d -> %d channels (need 8 -> 7)", cin, cout);
+    for (int c = 0; c < 15; ++c)
+        if (!((c < 8 ? in_std[c] : out_std[c - 8]) > 0.0))
+            return fail(h, MHD_ERR_ARG, "mhd_set_closure: standard deviations must be positive");
+    const Params &p = h->p;
+    const long long nvox = (long long)p.npz * p.plane;
+    int ndev = -1;
+    if (net_prepare(net, nvox, &ndev) != MHD_OK) return fail(h, MHD_ERR_HIP, "mhd_set_closure: network workspace");
+    if (ndev != h->dev) return fail(h, MHD_ERR_ARG, "mhd_set_closure: network on device %d, box on %d", ndev, h->dev);
+    const int nz = p.npz;
+    const size_t nin = 8 * (size_t)nz + 8, nout = 7 * (size_t)nz + 7;
+    if (!h->force) HIPCHK(h, hipMalloc(&h->force, (size_t)7 * p.fsize * sizeof(float)));
+    if (!h->force_on) HIPCHK(h, hipMalloc(&h->force_on, sizeof(int)));
+    if (!h->cx) HIPCHK(h, hipMalloc(&h->cx, (size_t)nvox * 8 * sizeof(__bf16)));
+    if (!h->nrm) HIPCHK(h, hipMalloc(&h->nrm, (nin + nout) * sizeof(float)));
+    std::vector<float> nrm(nin + nout);
+    for (size_t i = 0; i < 8 * (size_t)nz; ++i) nrm[i] = (float)in_profile[i];
+    for (int c = 0; c < 8; ++c) nrm[8 * nz + c] = (float)(1.0 / in_std[c]);
+    for (size_t i = 0; i < 7 * (size_t)nz; ++i) nrm[nin + i] = (float)out_profile[i];
+    for (int c = 0; c < 7; ++c) nrm[nin + 7 * nz + c] = (float)out_std[c];
+    HIPCHK(h, hipMemcpy(h->nrm, nrm.data(), nrm.size() * sizeof(float), hipMemcpyHostToDevice));
+    h->nrm_out = h->nrm + nin;
+    // forcing is zero outside the interior planes
+    HIPCHK(h, hipMemset(h->force, 0, (size_t)7 * p.fsize * sizeof(float)));
+    h->net = net;
+    h->closure = true;
+    if (h->have_bg) {
+        h->bg.force = h->force;
+        h->bg.force_on = h->force_on;
+    }
+    return MHD_OK;
+}
+
+extern "C" int mhd_closure_active(const mhd_t *h) { return h && h->closure ? 1 : 0; }
 
 extern "C" int mhd_box_filter(mhd_t *h, int factor, int zmode, int source, double *out) {
     int rc = ready(h, "mhd_box_filter");

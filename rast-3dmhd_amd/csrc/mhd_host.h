@@ -145,7 +145,7 @@ inline int background_tables(const Params &p, const double *dzzdz, const double 
 // Background table pointers into device blocks laid out as background_tables.
 inline Background background_view(int npz, const double *bgd, const float *bgf) {
     const int nz = npz + 2;
-    Background bg;
+    Background bg{};
     bg.ro = bgd;
     bg.tt = bgd + nz;
     bg.fwbg = bgd + 2 * nz;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+struct mhd_net;
+
 namespace rmhd {
 
 // Field order in every device register (q, Z, F).
@@ -60,6 +62,11 @@ struct DevScalars {
 struct Background {
     const double *ro, *tt, *fwbg, *ftbg;
     const float *ro32, *tt32, *dt0dz32, *rinv0_32, *s0_32;
+    // optional subgrid forcing of the closure-corrected step: FP32 fields
+    // [RU,RV,RW,TT,BX,BY,BZ][plane][j][i] added to the right-hand side of
+    // every stage while *force_on != 0 (null: none)
+    const float *force;
+    const int *force_on;
 };
 
 // One box of an ensemble as the batched kernels see it (device memory).
@@ -115,6 +122,16 @@ int stage_lds(int tile_y, int lmag, int mixed);
 hipError_t launch_box_filter(const Params &p, const double *q, int nf, int factor, int zmode, int nxc, int nyc, int nzc,
                              double *out, hipStream_t s);
 
+// Closure-corrected mixed step (mhd_mixed.hip): network inputs from the
+// state (rho, u, v, w, s = ln T/(gamma-1) - ln rho, BX, BY, BZ; nrm =
+// per-plane means [8][npz], 1/std[8]) as BF16 [plane-1][j][i][8], and the
+// forcing from the network output (nrm_out = per-plane mean subgrid terms
+// [7][npz], std[7]); force_on is cleared if any output is not finite.
+hipError_t launch_closure_inputs(const Params &p, const Background &bg, const float *h, const float *dr,
+                                 const float *nrm, __bf16 *x, hipStream_t s);
+hipError_t launch_closure_forcing(const Params &p, const float *y, const float *nrm_out, float *force, int *force_on,
+                                  hipStream_t s);
+
 // Mixed-precision form (mhd_mixed.hip).
 hipError_t launch_stage_mixed(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,
                               float *hout, float *dr, DevScalars *sc, hipStream_t s);
@@ -134,5 +151,10 @@ hipError_t launch_stage_batch_mixed(int mode, int tile_y, int lmag, const BoxDev
                                     int nitems, int grid, int a, hipStream_t s);
 hipError_t launch_bcon_batch_mixed(int lmag, const BoxDev *boxes, const ColumnItem *items, int nitems, int a,
                                    bool last, hipStream_t s);
+
+// Network entry points used by the closure-corrected step (mhd_net.hip).
+int net_shape(const mhd_net *n, int *cin, int *cout);
+int net_prepare(mhd_net *n, long long nvox, int *dev);
+hipError_t net_forward_device(mhd_net *n, hipStream_t s, int nx, int ny, int nz, const __bf16 *x, const float **y);
 
 }  // namespace rmhd

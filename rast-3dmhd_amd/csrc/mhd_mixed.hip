@@ -69,6 +69,83 @@ __host__ __device__ __forceinline__ double pprime(double r0, double t0, double r
     return r0 * tp + t0 * rh + rh * tp;
 }
 
+// CFL limits of the mixed form (FP64), accumulated in monotone transformed
+// form so that a point costs two divisions and one square root:
+//   min dd/sqrt(U)               = 1/sqrt(max U/dd^2)
+//   min 0.5 dd^2 REPR CV rho/kap = 0.5 REPR CV min(rho dd^2 / kap)
+//   min 0.375 dd^2 RE rho        = 0.375 RE min(rho dd^2)
+//   min 0.5 dd^2 RM              = 0.5 RM min(dd^2)
+// (U the fast-mode speed squared of sub:750-856).  The workgroup converts
+// back once and updates the same keys as cfl_reduce.  NaN sticks.
+struct CflFast {
+    double mach, q1, s2, s3, d2;
+};
+
+__device__ __forceinline__ double nmax(double a, double b) { return (b > a || !(b == b)) ? b : a; }
+__device__ __forceinline__ double nmin(double a, double b) { return (b < a || !(b == b)) ? b : a; }
+
+__device__ __forceinline__ CflFast cflf_init() { return CflFast{0.0, 0.0, INFINITY, INFINITY, INFINITY}; }
+
+template <bool MAG>
+__device__ __forceinline__ void cflf_point(const Params &p, const double *q, double dd2, double idd2, double irk,
+                                           CflFast &c) {
+    const double ro = q[F_RO], tt = q[F_TT];
+    const double ri = 1.0 / ro;
+    const double uu = (q[F_RU] * q[F_RU] + q[F_RV] * q[F_RV] + q[F_RW] * q[F_RW]) * (ri * ri);
+    c.mach = nmax(c.mach, p.ogamma * uu / tt);
+    double cs = p.gamma * tt;
+    if (MAG) cs += (q[F_BX] * q[F_BX] + q[F_BY] * q[F_BY] + q[F_BZ] * q[F_BZ]) * p.obeta * ri;
+    const double u2 = uu + cs + 2.0 * sqrt(uu * cs);
+    c.q1 = nmax(c.q1, u2 * idd2);
+    const double rd = ro * dd2;
+    c.s2 = nmin(c.s2, rd * irk);
+    c.s3 = nmin(c.s3, rd);
+    if (MAG) c.d2 = nmin(c.d2, dd2);
+}
+
+// dd^2 = min(ddx, ddy, ddz)^2 and its inverse for one point
+struct DdPair {
+    double dd2, idd2;
+};
+__device__ __forceinline__ DdPair ddpair(double ddx, double ddy, double ddz) {
+    const double a = ddx * ddx, b = ddy * ddy, c = ddz * ddz;
+    const double d2 = fmin(fmin(a, b), c);
+    return DdPair{d2, 1.0 / d2};
+}
+
+__device__ void cflf_reduce(const Params &p, CflFast c, DevScalars *__restrict__ sc) {
+    __shared__ double red[16][5];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        c.mach = nmax(c.mach, __shfl_xor(c.mach, off, 64));
+        c.q1 = nmax(c.q1, __shfl_xor(c.q1, off, 64));
+        c.s2 = nmin(c.s2, __shfl_xor(c.s2, off, 64));
+        c.s3 = nmin(c.s3, __shfl_xor(c.s3, off, 64));
+        c.d2 = nmin(c.d2, __shfl_xor(c.d2, off, 64));
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) {
+        red[wv][0] = c.mach; red[wv][1] = c.q1; red[wv][2] = c.s2; red[wv][3] = c.s3; red[wv][4] = c.d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 1; v < nw; ++v) {
+            c.mach = nmax(c.mach, red[v][0]);
+            c.q1 = nmax(c.q1, red[v][1]);
+            c.s2 = nmin(c.s2, red[v][2]);
+            c.s3 = nmin(c.s3, red[v][3]);
+            c.d2 = nmin(c.d2, red[v][4]);
+        }
+        const double w[4] = {1.0 / sqrt(c.q1), 0.5 * p.repr * p.cv * c.s2, 0.375 * p.re * c.s3, 0.5 * p.rm * c.d2};
+        atomicMax(&sc->key_mach, dkey(c.mach));
+        for (int n = 0; n < 4; ++n) {
+            if (n == 3 && !p.lmag) break;
+            // a NaN limit becomes -NaN for the min reduction
+            atomicMin(&sc->key_min[n], (w[n] == w[n]) ? dkey(w[n]) : 0ull);
+        }
+    }
+}
+
 struct RawF {
     float v[NFIELD];
 };
@@ -160,8 +237,9 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
     const double dt = sc->dt;
     const double coefg = (MODE == 1 ? p.gam1 : MODE == 2 ? p.gam2 : p.gam3) * dt;
     const double coefz = (MODE == 1 ? p.zeta1 : p.zeta2) * dt;
-    double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
+    CflFast cfl = cflf_init();
     const double c_ddx = MODE == 3 ? m.ddx[gix] : 0.0, c_ddy = MODE == 3 ? m.ddy[gjy] : 0.0;
+    const bool frc = bg.force != nullptr && *bg.force_on != 0;
 
 #pragma unroll 1
     for (int k = kbeg; k <= kend; ++k) {
@@ -174,7 +252,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         float dk_[8];
         if (valid) {
 #pragma unroll
-            for (int f = 0; f < NF; ++f) dk_[f] = dr[f * fs + o];
+            for (int f = 0; f < NF; ++f) dk_[f] = __builtin_nontemporal_load(&dr[f * fs + o]);  // read once
         }
         __syncthreads();
         if (valid) {
@@ -339,6 +417,18 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             double Fd[8] = {fu, fv, fw, fr, ft, w1, w2, w3};
             Fd[2] += bg.fwbg[k];
             Fd[4] += bg.ftbg[k];
+            if (frc) {
+                const float *g = bg.force + o;
+                Fd[0] += (double)g[0];
+                Fd[1] += (double)g[fs];
+                Fd[2] += (double)g[2 * fs];
+                Fd[4] += (double)g[3 * fs];
+                if (MAG) {
+                    Fd[5] += (double)g[4 * fs];
+                    Fd[6] += (double)g[5 * fs];
+                    Fd[7] += (double)g[6 * fs];
+                }
+            }
             const double r0d = bg.ro[k], t0d = bg.tt[k];
             double xq[8];
             float hn[8];
@@ -362,7 +452,8 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 for (int f = 0; f < NF; ++f) full[f] = xq[f];
                 full[3] += bg.ro[k];
                 full[4] += bg.tt[k];
-                cfl_point<MAG>(p, full, c_ddx, c_ddy, m.ddz[k], m.rkapa[k], c_mach, c_w1, c_w2, c_w3, c_w4);
+                const DdPair d = ddpair(c_ddx, c_ddy, m.ddz[k]);
+                cflf_point<MAG>(p, full, d.dd2, d.idd2, 1.0 / m.rkapa[k], cfl);
             }
 #pragma unroll
             for (int f = 0; f < NF; ++f) rm_.v[f] = h0[f];
@@ -377,7 +468,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             }
         }
     }
-    if (MODE == 3) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
+    if (MODE == 3) cflf_reduce(p, cfl, sc);
 }
 
 template <int TY, bool MAG, int MODE>
@@ -420,7 +511,7 @@ template <bool MAG, bool LAST>
 __device__ __forceinline__ void bcon_column_m(const Params &p, const Metrics &m, const Background &bg,
                                               float *__restrict__ h, float *__restrict__ dr,
                                               DevScalars *__restrict__ sc, int idx) {
-    double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
+    CflFast cfl = cflf_init();
     if (idx < p.npx * p.npy) {
         const long long fs = p.fsize, pl = p.plane;
         const int nz = p.npz;
@@ -473,12 +564,12 @@ __device__ __forceinline__ void bcon_column_m(const Params &p, const Metrics &m,
                 const int k = side ? nz : 1;
                 double v[8];
                 for (int f = 0; f < (MAG ? 8 : 5); ++f) v[f] = full(f, k);
-                cfl_point<MAG>(p, v, m.ddx[i + 1], m.ddy[j + 1], m.ddz[k], m.rkapa[k], c_mach, c_w1, c_w2,
-                                 c_w3, c_w4);
+                const DdPair d = ddpair(m.ddx[i + 1], m.ddy[j + 1], m.ddz[k]);
+                cflf_point<MAG>(p, v, d.dd2, d.idd2, 1.0 / m.rkapa[k], cfl);
             }
         }
     }
-    if (LAST) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
+    if (LAST) cflf_reduce(p, cfl, sc);
 }
 
 template <bool MAG, bool LAST>
@@ -504,7 +595,7 @@ __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, 
                                                DevScalars *__restrict__ sc) {
     const long long fs = p.fsize, pl = p.plane;
     const long long n = (long long)p.npz * pl;
-    double mach = 0.0, w1 = INFINITY, w2 = INFINITY, w3 = INFINITY, w4 = INFINITY;
+    CflFast cfl = cflf_init();
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long long)gridDim.x * blockDim.x) {
         const long long o = pl + idx;
@@ -518,9 +609,10 @@ __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, 
         }
         v[3] += bg.ro[k];
         v[4] += bg.tt[k];
-        cfl_point<MAG>(p, v, m.ddx[i + 1], m.ddy[j + 1], m.ddz[k], m.rkapa[k], mach, w1, w2, w3, w4);
+        const DdPair d = ddpair(m.ddx[i + 1], m.ddy[j + 1], m.ddz[k]);
+        cflf_point<MAG>(p, v, d.dd2, d.idd2, 1.0 / m.rkapa[k], cfl);
     }
-    cfl_reduce(mach, w1, w2, w3, w4, p.lmag, sc);
+    cflf_reduce(p, cfl, sc);
 }
 
 // padded FP64 field f -> (stored value, D = rounding residual) in the
@@ -568,6 +660,53 @@ __global__ void k_unpack_m(const Params p, const float *__restrict__ hreg, const
         if (field == 4) v = bg.tt[k] + (double)trep(bg.ro32[k], bg.tt32[k], hreg[3 * fs + o], hreg[4 * fs + o]);
         else v = (double)hreg[field * fs + o] + (field == 3 ? bg.ro[k] : 0.0);
         padded[idx] = v + (double)dreg[field * fs + o];
+    }
+}
+
+// Network inputs of the closure from the state (h, D) about the background.
+__global__ void k_closure_inputs(const Params p, const Background bg, const float *__restrict__ h,
+                                 const float *__restrict__ dr, const float *__restrict__ nrm, __bf16 *__restrict__ x) {
+    const long long fs = p.fsize, n = (long long)p.npz * p.plane;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(v / p.plane) + 1;
+        const long long o = p.plane + v;
+        auto val = [&](int f) { return (double)h[f * fs + o] + (double)dr[f * fs + o]; };
+        const double ro = bg.ro[k] + val(3);
+        const double tt = bg.tt[k] + (double)trep(bg.ro32[k], bg.tt32[k], h[3 * fs + o], h[4 * fs + o]) +
+                          (double)dr[4 * fs + o];
+        const double ri = 1.0 / ro;
+        double c[8];
+        c[0] = ro;
+        c[1] = val(0) * ri;
+        c[2] = val(1) * ri;
+        c[3] = val(2) * ri;
+        c[4] = log(tt) / (p.gamma - 1.0) - log(ro);
+        for (int f = 5; f < 8; ++f) c[f] = p.lmag ? val(f) : 0.0;
+        // nrm: per-plane means [8][npz], then 1/std [8]
+        for (int f = 0; f < 8; ++f)
+            x[v * 8 + f] = (__bf16)(float)((c[f] - (double)nrm[f * p.npz + k - 1]) * (double)nrm[8 * p.npz + f]);
+    }
+}
+
+__global__ void k_set_flag(int *flag) { *flag = 1; }
+
+// Forcing fields from the network output y[v][8] (7 channels used).
+__global__ void k_closure_forcing(const Params p, const float *__restrict__ y, const float *__restrict__ nrm,
+                                  float *__restrict__ force, int *__restrict__ force_on) {
+    const long long fs = p.fsize, n = (long long)p.npz * p.plane;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x) {
+        const long long o = p.plane + v;
+        const int k = (int)(v / p.plane);  // plane index 0..npz-1
+        bool ok = true;
+        // nrm: per-plane mean subgrid terms [7][npz], then std [7]
+        for (int c = 0; c < 7; ++c) {
+            const float g = y[v * 8 + c] * nrm[7 * p.npz + c] + nrm[c * p.npz + k];
+            ok = ok && (g - g == 0.0f);
+            force[c * fs + o] = g;
+        }
+        if (!ok) *force_on = 0;
     }
 }
 
@@ -709,6 +848,25 @@ hipError_t launch_bcon_batch_mixed(int lmag, const BoxDev *boxes, const ColumnIt
         if (last) hipLaunchKernelGGL((k_bcon_m_batch<false, true>), grid, block, 0, s, boxes, items, a);
         else hipLaunchKernelGGL((k_bcon_m_batch<false, false>), grid, block, 0, s, boxes, items, a);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_closure_inputs(const Params &p, const Background &bg, const float *h, const float *dr,
+                                 const float *nrm, __bf16 *x, hipStream_t s) {
+    long long nb = ((long long)p.npz * p.plane + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_closure_inputs, dim3((unsigned)nb), dim3(256), 0, s, p, bg, h, dr, nrm, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_closure_forcing(const Params &p, const float *y, const float *nrm_out, float *force, int *force_on,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, s, force_on);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    long long nb = ((long long)p.npz * p.plane + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(k_closure_forcing, dim3((unsigned)nb), dim3(256), 0, s, p, y, nrm_out, force, force_on);
     return hipGetLastError();
 }
 

@@ -492,19 +492,21 @@ static hipError_t pack_weights(mhd_net_t *n) {
     return hipSuccess;
 }
 
-// forward of the BF16 input a0 (stride 8) into y (FP32, stride 8)
-static hipError_t forward(mhd_net_t *n, int nx, int ny, int nz) {
+// forward of a BF16 input x (stride 8) into n->y (FP32, stride 8) on stream s
+static hipError_t forward_on(mhd_net_t *n, hipStream_t s, const bf16 *x, int nx, int ny, int nz) {
     const Layer *L = n->L;
-    hipError_t e = conv<8, 2, 1>(n->stream, n->a0, L[0].wf, L[0].kpf, n->p + L[0].boff, nullptr, n->a1, nullptr, n->z1,
-                                 L[0].co, L[0].cos_, nx, ny, nz);
+    hipError_t e = conv<8, 2, 1>(s, x, L[0].wf, L[0].kpf, n->p + L[0].boff, nullptr, n->a1, nullptr, n->z1, L[0].co,
+                                 L[0].cos_, nx, ny, nz);
     if (e == hipSuccess)
-        e = conv<32, 2, 1>(n->stream, n->a1, L[1].wf, L[1].kpf, n->p + L[1].boff, nullptr, n->a2, nullptr, n->z2,
-                           L[1].co, L[1].cos_, nx, ny, nz);
+        e = conv<32, 2, 1>(s, n->a1, L[1].wf, L[1].kpf, n->p + L[1].boff, nullptr, n->a2, nullptr, n->z2, L[1].co,
+                           L[1].cos_, nx, ny, nz);
     if (e == hipSuccess)
-        e = conv<32, 1, 0>(n->stream, n->a2, L[2].wf, L[2].kpf, n->p + L[2].boff, nullptr, nullptr, n->y, nullptr,
-                           L[2].co, 8, nx, ny, nz);
+        e = conv<32, 1, 0>(s, n->a2, L[2].wf, L[2].kpf, n->p + L[2].boff, nullptr, nullptr, n->y, nullptr, L[2].co, 8,
+                           nx, ny, nz);
     return e;
 }
+
+static hipError_t forward(mhd_net_t *n, int nx, int ny, int nz) { return forward_on(n, n->stream, n->a0, nx, ny, nz); }
 
 template <int CI, int CO>
 static hipError_t dw(mhd_net_t *n, const bf16 *x, const Layer &L, const bf16 *g, int nx, int ny, int nz) {
@@ -823,3 +825,32 @@ extern "C" void mhd_net_destroy(mhd_net_t *n) {
     if (n->stream) (void)hipStreamDestroy(n->stream);
     delete n;
 }
+
+// Device-side entry points for the closure-corrected step (mhd_capi.hip).
+namespace rmhd {
+
+int net_shape(const mhd_net_t *n, int *cin, int *cout) {
+    if (!n) return MHD_ERR_ARG;
+    *cin = n->cin;
+    *cout = n->cout;
+    return MHD_OK;
+}
+
+// Workspace for volumes of nvox points and the device the network lives on.
+int net_prepare(mhd_net_t *n, long long nvox, int *dev) {
+    *dev = n->dev;
+    const int rc = ensure_workspace(n, nvox);
+    if (rc) return rc;
+    return hipStreamSynchronize(n->stream) == hipSuccess ? MHD_OK : MHD_ERR_HIP;
+}
+
+// y = net(x) for a BF16 input (stride 8) resident on the device, queued on
+// stream s; *y points to the output (FP32, stride 8) in the network's
+// workspace, valid until the next use of the network.
+hipError_t net_forward_device(mhd_net_t *n, hipStream_t s, int nx, int ny, int nz, const __bf16 *x,
+                              const float **y) {
+    *y = n->y;
+    return forward_on(n, s, x, nx, ny, nz);
+}
+
+}  // namespace rmhd

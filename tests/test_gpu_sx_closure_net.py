@@ -75,11 +75,11 @@ def test_training_reduces_loss():
         for x, t in zip(xs, ts):
             net.add_sample(x, t)
         l0 = net.loss(0)
-        first, last = net.train(1200, 3e-3)
+        first, last = net.train(600, 2e-3)
         l1 = net.loss(0)
         say("loss", l0, first, last, l1)
     assert first == pytest.approx(l0, rel=1e-6)
-    assert l1 < 0.1 * l0
+    assert l1 < 0.25 * l0
 
 
 @pytest.mark.parametrize("npz,zmode", [(17, 1), (16, 0)])
@@ -99,3 +99,70 @@ def test_box_filter_matches_oracle(npz, zmode):
     for f, k in enumerate(["fu", "fv", "fw", "fr", "ft", "ww1", "ww2", "ww3"]):
         ref = filt(rhs[k][1:-1, 1:-1, 1:-1].transpose(2, 1, 0), 4)
         assert np.allclose(got_rhs[f], ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max()), k
+
+
+def _closure_case():
+    import importlib
+    wl = importlib.import_module("rast-3dmhd_amd.workload")
+    return wl.stratified_box(64, 48, 24, seed=3)
+
+
+def _run_mixed(meta, st, nsteps, net=None, val_error=0.0, threshold=0.1, out_std=None):
+    with H.pkg().Box(meta, precision="mixed") as box:
+        box.upload(st)
+        if net is not None:
+            nz = int(meta["npz"])
+            box.set_closure(net, np.zeros((8, nz)), np.ones(8), np.zeros((7, nz)),
+                            np.ones(7) if out_std is None else out_std, val_error, threshold)
+        dt, _ = box.steps(nsteps)
+        return box.download(), dt, (box.closure_active() if net is not None else False)
+
+
+def test_closure_fallbacks_step_plainly():
+    """Rejected network (error above threshold), all-zero network, and a
+    network with non-finite output: every one steps bit-identically to the
+    plain mixed step."""
+    say("case closure fallbacks")
+    meta, st = _closure_case()
+    ref, dt_ref, _ = _run_mixed(meta, st, 3)
+    with H.pkg().Net(CIN, WIDTH, COUT) as net:
+        net.set_params(N.flatten(N.init_params(CIN, WIDTH, COUT, seed=5)))
+        got, dt, active = _run_mixed(meta, st, 3, net, val_error=0.5, threshold=0.1)
+        assert not active and dt == dt_ref
+        for k in H.FIELDS:
+            assert np.array_equal(got[k], ref[k]), ("rejected", k)
+        net.set_params(np.zeros(net.nparam, dtype=np.float32))
+        got, dt, active = _run_mixed(meta, st, 3, net)
+        assert active and dt == dt_ref
+        for k in H.FIELDS:
+            assert np.array_equal(got[k], ref[k]), ("zero", k)
+        bad = np.zeros(net.nparam, dtype=np.float32)
+        bad[-COUT:] = np.nan  # output biases
+        net.set_params(bad)
+        got, dt, active = _run_mixed(meta, st, 3, net)
+        assert active and dt == dt_ref
+        for k in H.FIELDS:
+            assert np.array_equal(got[k], ref[k]), ("nan", k)
+
+
+def test_closure_constant_forcing_adds_dt_g():
+    """Only output biases set: a constant forcing g per field; one step adds
+    dt * g (the RK3 weights sum to one) up to the coupling within the step."""
+    say("case closure constant forcing")
+    meta, st = _closure_case()
+    ref, dt_ref, _ = _run_mixed(meta, st, 1)
+    g = np.array([1e-3, -2e-3, 5e-4, 1e-3, 1e-6, -1e-6, 0.0])
+    with H.pkg().Net(CIN, WIDTH, COUT) as net:
+        p = np.zeros(net.nparam, dtype=np.float32)
+        p[-COUT:] = np.sign(g)  # output = sign(g); de-normalised by |g|
+        net.set_params(p)
+        got, dt, active = _run_mixed(meta, st, 1, net, out_std=np.where(g == 0, 1.0, np.abs(g)))
+    assert active and dt == dt_ref
+    # away from the walls: next to them a uniform RW forcing meets RW = 0 at
+    # the boundary plane and the acoustic response within the step is
+    # O(CFL^2), a few per cent of dt * g
+    inner = (slice(1, -1), slice(1, -1), slice(3, -3))
+    for f, k in enumerate(["ru", "rv", "rw", "tt", "bx", "by"]):
+        d = (got[k] - ref[k])[inner]
+        want = dt * g[f]
+        assert np.abs(d - want).max() <= 1e-2 * abs(want), (k, np.abs(d - want).max(), want)
