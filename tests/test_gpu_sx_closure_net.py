@@ -166,3 +166,75 @@ def test_closure_constant_forcing_adds_dt_g():
         d = (got[k] - ref[k])[inner]
         want = dt * g[f]
         assert np.abs(d - want).max() <= 1e-2 * abs(want), (k, np.abs(d - want).max(), want)
+
+
+def test_closure_forcing_matches_host_evaluation():
+    """Network inputs built on the device (per-plane input profiles, input
+    scales), network output de-normalised with per-plane output profiles:
+    with a fixed, tiny dt the step changes each forced field by dt * g, g
+    evaluated on the host from the same state with the same network."""
+    import importlib
+    say("case closure forcing vs host evaluation")
+    cl = importlib.import_module("rast-3dmhd_amd.closure")
+    meta, st = _closure_case()
+    # the case starts at rest with horizontally uniform density: add motion
+    # and density fluctuations so that every input channel varies
+    rng = np.random.default_rng(21)
+    inner3 = (slice(1, -1), slice(1, -1), slice(1, -1))
+    st["ro"][inner3] *= 1.0 + 1e-3 * rng.standard_normal(st["ro"][inner3].shape)
+    for k in ("ru", "rv", "rw"):
+        st[k][inner3] = st["ro"][inner3] * 1e-2 * rng.standard_normal(st[k][inner3].shape)
+    st["rw"][:, :, 1] = st["rw"][:, :, -2] = 0.0
+    for k in ("ru", "rv", "rw", "ro"):
+        H.ics().periodic_fill(st[k])
+    _, dt0, _ = _run_mixed(meta, st, 1)
+    dt = dt0 * 1e-3
+    nz = int(meta["npz"])
+    x = cl.inputs_of(cl.interior(st), float(meta["gamma"]))
+    in_profile = x.mean(axis=(1, 2)).T.copy()
+    in_std = (x - in_profile.T[:, None, None, :]).std(axis=(0, 1, 2))
+    rms = np.array([np.sqrt(np.mean(st[k][1:-1, 1:-1, 1:-1] ** 2)) for k in cl.TARGETS])
+    out_std = 1e-3 * rms / dt
+    kk = np.arange(nz) / (nz - 1)
+    out_profile = np.stack([0.5 * out_std[c] * np.cos(np.pi * kk + c) for c in range(7)])
+    with H.pkg().Net(CIN, WIDTH, COUT) as net:
+        net.set_params(N.flatten(N.init_params(CIN, WIDTH, COUT, seed=9)))
+        nrm = {"in_profile": in_profile, "in_std": in_std, "out_profile": out_profile, "out_std": out_std}
+        g = cl.Experiment.decode(net.forward(cl.Experiment.encode(x, nrm)), nrm)  # [K][J][I][7]
+        outs = []
+        for attach in (False, True):
+            with H.pkg().Box(meta, precision="mixed") as box:
+                box.upload(st)
+                box.set_fixed_dt(dt)
+                if attach:
+                    box.set_closure(net, in_profile, in_std, out_profile, out_std, 0.0, 0.1)
+                    assert box.closure_active()
+                box.steps(1)
+                outs.append(box.download())
+    inner = (slice(1, -1), slice(1, -1), slice(3, -3))
+    for c, k in enumerate(cl.TARGETS):
+        d = (outs[1][k] - outs[0][k])[inner]
+        want = dt * g[..., c].transpose(2, 1, 0)[:, :, 2:-2]
+        err = np.abs(d - want).max() / np.abs(want).max()
+        say(k, err)
+        assert err <= 5e-3, (k, err)
+
+
+def test_closure_pipeline_beats_plain_coarse():
+    """Whole pipeline at coarse 32x32x32 against filtered fine 128x128x125:
+    the network trained on the GPU passes the held-out and a-posteriori
+    checks, is attached, and the closure-corrected coarse run reproduces the
+    filtered fine run's mean convective-flux profile and kinetic-energy
+    spectrum better than the plain coarse run."""
+    import importlib
+    say("case closure pipeline 32x32x32")
+    cl = importlib.import_module("rast-3dmhd_amd.closure")
+    r = cl.run(32, 32, 32, until_mach=0.05, develop=2000, nsamples=24, every=200, horizon=1600, nsnap=8,
+               iters=6000, log=say)
+    say({k: r[k] for k in ("held_out_error", "closure_active", "flux_error_plain", "flux_error_closure",
+                           "ke_spectrum_error_plain", "ke_spectrum_error_closure")})
+    assert r["held_out_error"] <= 0.1 and r["aposteriori_validation"]["closure_better"]
+    assert r["closure_active"]
+    assert r["flux_error_closure"] < 0.5 * r["flux_error_plain"]
+    assert r["flux_error_closure"] <= 0.05
+    assert r["ke_spectrum_error_closure"] < r["ke_spectrum_error_plain"]
