@@ -5,6 +5,9 @@ module rast3dmhd_c
   use iso_c_binding
   implicit none
 
+  ! mhd_desc%precision
+  integer(c_int), parameter :: MHD_FP64 = 0, MHD_MIXED = 1
+
   type, bind(c) :: mhd_desc
     integer(c_int) :: npx, npy, npz
     integer(c_int) :: lmag, lrot
@@ -58,6 +61,12 @@ module rast3dmhd_c
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(out) :: dt, umach
+    end function
+
+    integer(c_int) function mhd_set_fixed_dt(h, dt) bind(c, name='mhd_set_fixed_dt')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: dt
     end function
 
     integer(c_int) function mhd_set_clock(h, nit, timc, timi) bind(c, name='mhd_set_clock')

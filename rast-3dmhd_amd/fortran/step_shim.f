@@ -71,6 +71,8 @@ C
       TYPE(MHD_DESC) :: D
       INTEGER(C_INT) :: IRC
       REAL(C_DOUBLE) :: DTC, UMC
+      CHARACTER(LEN=16) :: PREC
+      INTEGER :: PLEN, PST
 C
       IF (.NOT.READY) THEN
          IF (NPEY*NPEZ.NE.1) THEN
@@ -91,7 +93,18 @@ C
          D%IZCON=IZC
          D%ITCON=ITC
          D%IBCON=IBC
-         D%PRECISION=0
+C        RAST3DMHD_PRECISION=mixed selects the mixed-precision form
+         D%PRECISION=MHD_FP64
+         CALL GET_ENVIRONMENT_VARIABLE('RAST3DMHD_PRECISION',PREC,
+     2                                 PLEN,PST)
+         IF ((PST.EQ.0).AND.(PLEN.GT.0)) THEN
+            IF (PREC(1:PLEN).EQ.'mixed') THEN
+               D%PRECISION=MHD_MIXED
+            ELSE IF (PREC(1:PLEN).NE.'fp64') THEN
+               WRITE(6,*)'STEP: RAST3DMHD_PRECISION is fp64 or mixed'
+               STOP
+            ENDIF
+         ENDIF
          D%HX=HX
          D%H2X=H2X
          D%HY=HY

@@ -73,3 +73,47 @@ def test_fortran_dropin_matches_reference(tmp_path, name):
     assert dt == float(g[f"step{n}_dt"]) and umach == float(g[f"step{n}_umach"])
     assert timc == float(g[f"step{n}_timc"]) and timt == float(g[f"step{n}_timt"])
     assert H.state_sha(st) == str(g[f"step{n}_sha"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["hydro16", "mhdrot16"])
+def test_fortran_dropin_mixed_form(tmp_path, name):
+    """RAST3DMHD_PRECISION=mixed: the same driver and shim step the
+    mixed-precision form; one step agrees with the reference's FP64 step to
+    1e-6 relative L2 per field (the B components against the field's RMS
+    magnitude), and DT, computed from the same state in FP64, to 1e-12."""
+    print("case fortran drop-in mixed", name, flush=True)
+    exe = os.path.join(FBUILD, name, "rastreplay")
+    assert os.path.exists(exe), "build the Fortran drop-in first (__graft_entry__.build())"
+    g = H.load(name)
+    write_replay_input(tmp_path / "replay.in", g, H.initial_state(name), 1)
+    env = dict(os.environ, RAST3DMHD_PRECISION="mixed")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    st, dt, umach, timc, timt, nit = read_replay_output(tmp_path / "replay.out", g)
+    assert nit == 1
+    assert abs(dt - float(g["step1_dt"])) <= 1e-12 * float(g["step1_dt"])
+    assert abs(umach - float(g["step1_umach"])) <= 1e-6 * float(g["step1_umach"])
+    inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+    lmag = int(g["lmag"])
+    bscale = np.sqrt(sum(np.mean(np.asarray(g[f"step1_{k}"])[inner] ** 2) for k in ("bx", "by", "bz"))) if lmag else 0
+    for k in (H.FIELDS if lmag else H.FIELDS[:5]):
+        ref = np.asarray(g[f"step1_{k}"])[inner]
+        d = np.sqrt(np.mean((st[k][inner] - ref) ** 2))
+        scale = bscale if k in ("bx", "by", "bz") else np.sqrt(np.mean(ref ** 2))
+        print(k, d / scale, flush=True)
+        assert d <= 1e-6 * scale, (k, d / scale)
+
+
+def test_shim_rejects_unknown_precision(tmp_path):
+    """The shim's precision switch is read on the first STEP call, before any
+    GPU work: an unknown value stops the driver with a message."""
+    exe = os.path.join(FBUILD, "hydro16", "rastreplay")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drop-in not built")
+    g = H.load("hydro16")
+    write_replay_input(tmp_path / "replay.in", g, H.initial_state("hydro16"), 1)
+    env = dict(os.environ, RAST3DMHD_PRECISION="half")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
+    assert "RAST3DMHD_PRECISION is fp64 or mixed" in r.stdout + r.stderr
+    assert not (tmp_path / "replay.out").exists()
