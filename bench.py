@@ -41,7 +41,9 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # algorithmic HBM bytes per zone per RK3 step (8 fields, one kernel per stage)
 BYTES_PER_ZONE_STEP = {
     "fp64": 640.0,   # q read + Z read/write + q write: 10 accesses x 8 B
-    "mixed": 384.0,  # per stage h read + D read + h write + D write: 4 x 4 B
+    # stage 1: h, R (BF16) read, h, D written; stage 2: h, D read and written;
+    # stage 3: h, D read, h, R written: (14 + 16 + 14) B x 8 fields
+    "mixed": 352.0,
 }
 KERNEL = {"fp64": "k_stage (fused RHS + RK3 stage, FP64)",
           "mixed": "k_stage_m (fused FP32 perturbation RHS + FP64 RK3 accumulation)"}

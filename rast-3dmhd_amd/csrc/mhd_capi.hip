@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -32,7 +33,8 @@ struct mhd_ctx {
     double *q[2] = {nullptr, nullptr};  // FP64: ping-pong state registers
     double *z = nullptr;                // FP64: RK3 Z register
     float *hq[2] = {nullptr, nullptr};  // mixed: FP32 perturbation registers
-    float *dr = nullptr;                // mixed: FP32 increment/residual register
+    float *dr = nullptr;                // mixed: FP32 increment/residual register (within a step)
+    unsigned *rr = nullptr;             // mixed: BF16 residual pairs (between steps)
     double *bgd = nullptr;              // mixed: FP64 background arrays
     float *bgf = nullptr;               // mixed: FP32 background arrays
     Background bg{};
@@ -81,7 +83,7 @@ static void free_all(mhd_t *h) {
     if (!h) return;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *bufs[16] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->bgd, h->bgf,
+    void *bufs[17] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->rr, h->bgd, h->bgf,
                       h->f, h->stage, h->met, h->sc, h->force, h->force_on, h->cx, h->nrm};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -122,6 +124,7 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
         if (e == hipSuccess) e = hipMalloc(&h->hq[0], fbytes);
         if (e == hipSuccess) e = hipMalloc(&h->hq[1], fbytes);
         if (e == hipSuccess) e = hipMalloc(&h->dr, fbytes);
+        if (e == hipSuccess) e = hipMalloc(&h->rr, resid_bytes(nf, p.fsize));
         if (e == hipSuccess) e = hipMalloc(&h->bgd, 4 * nzp * sizeof(double));
         if (e == hipSuccess) e = hipMalloc(&h->bgf, 5 * nzp * sizeof(float));
     }
@@ -162,6 +165,10 @@ extern "C" int mhd_set_metrics(mhd_t *h, const double *dxxdx, const double *d2xx
         off += len[n];
     }
     h->m = mt;
+    // the grid-only CFL term, which the stage kernels take from here
+    const double d2 = dd2_min(h->p, ddx, ddy, ddz);
+    HIPCHK(h, hipMemcpyAsync(reinterpret_cast<char *>(h->sc) + offsetof(DevScalars, dd2min), &d2, sizeof(d2),
+                             hipMemcpyHostToDevice, h->stream));
     const size_t nz = (size_t)h->p.npz + 2;
     h->dzzdz.assign(dzzdz, dzzdz + nz);
     h->d2zzdz2.assign(d2zzdz2, d2zzdz2 + nz);
@@ -213,8 +220,8 @@ extern "C" int mhd_upload(mhd_t *h, const double *ru, const double *rv, const do
             HIPCHK(h, launch_pack(h->p, h->stage, h->q[0] + off, h->stream));
             HIPCHK(h, launch_pack(h->p, h->stage, h->q[1] + off, h->stream));
         } else {
-            HIPCHK(h, launch_pack_mixed(h->p, h->stage, f, h->bg, h->hq[1], h->dr, h->stream));
-            HIPCHK(h, launch_pack_mixed(h->p, h->stage, f, h->bg, h->hq[0], h->dr, h->stream));
+            HIPCHK(h, launch_pack_mixed(h->p, h->stage, f, h->bg, h->hq[1], h->rr, h->stream));
+            HIPCHK(h, launch_pack_mixed(h->p, h->stage, f, h->bg, h->hq[0], h->rr, h->stream));
         }
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -236,7 +243,7 @@ extern "C" int mhd_download(mhd_t *h, double *ru, double *rv, double *rw, double
         if (!dst[f]) continue;
         const size_t off = (size_t)f * h->p.fsize;
         if (!h->mixed) HIPCHK(h, launch_unpack(h->p, h->q[h->cur] + off, h->stage, 1, h->stream));
-        else HIPCHK(h, launch_unpack_mixed(h->p, h->hq[h->cur], h->dr, f, h->bg, h->stage, h->stream));
+        else HIPCHK(h, launch_unpack_mixed(h->p, h->hq[h->cur], h->rr, f, h->bg, h->stage, h->stream));
         HIPCHK(h, hipMemcpyAsync(dst[f], h->stage, pbytes, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
     }
@@ -257,13 +264,13 @@ static hipError_t launches(mhd_t *h, hipEvent_t *ev) {
     rec(0);
     if (!h->cfl_valid) {
         if (!h->mixed) e = launch_cfl(p, h->m, h->q[h->cur], h->sc, h->stream);
-        else e = launch_cfl_mixed(p, h->m, h->bg, h->hq[h->cur], h->dr, h->sc, h->stream);
+        else e = launch_cfl_mixed(p, h->m, h->bg, h->hq[h->cur], h->rr, h->sc, h->stream);
         if (e == hipSuccess) e = launch_cfl_final(p, h->sc, false, h->stream);
     }
     if (e == hipSuccess && h->closure) {
         // subgrid forcing from the network on the state at the start of the step
         const float *y = nullptr;
-        e = launch_closure_inputs(p, h->bg, h->hq[h->cur], h->dr, h->nrm, h->cx, h->stream);
+        e = launch_closure_inputs(p, h->bg, h->hq[h->cur], h->rr, h->nrm, h->cx, h->stream);
         if (e == hipSuccess) e = net_forward_device(h->net, h->stream, p.npx, p.npy, p.npz, h->cx, &y);
         if (e == hipSuccess) e = launch_closure_forcing(p, y, h->nrm_out, h->force, h->force_on, h->stream);
     }
@@ -272,12 +279,12 @@ static hipError_t launches(mhd_t *h, hipEvent_t *ev) {
     for (int st = 1; st <= 3 && e == hipSuccess; ++st) {
         rec(2 + 4 * (st - 1));
         if (!h->mixed) e = launch_stage(st, p, h->m, h->q[a], h->q[b], h->z, nullptr, h->sc, h->stream);
-        else e = launch_stage_mixed(st, p, h->m, h->bg, h->hq[a], h->hq[b], h->dr, h->sc, h->stream);
+        else e = launch_stage_mixed(st, p, h->m, h->bg, h->hq[a], h->hq[b], h->dr, h->rr, h->sc, h->stream);
         rec(3 + 4 * (st - 1));
         rec(4 + 4 * (st - 1));
         if (e == hipSuccess) {
             if (!h->mixed) e = launch_bcon(p, h->m, h->q[a], h->q[b], h->sc, st == 3, h->stream);
-            else e = launch_bcon_mixed(p, h->m, h->bg, h->hq[b], h->dr, h->sc, st == 3, h->stream);
+            else e = launch_bcon_mixed(p, h->m, h->bg, h->hq[b], h->dr, h->rr, h->sc, st == 3, h->stream);
         }
         rec(5 + 4 * (st - 1));
         const int t = a;

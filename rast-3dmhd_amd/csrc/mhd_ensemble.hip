@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -175,7 +176,7 @@ extern "C" int mhd_ens_create(const mhd_desc *descs, int nbox, mhd_ens_t **out) 
         B.p = params_from_desc(&descs[b]);
         B.reg_elems = (size_t)nf * B.p.fsize;
         reg_off[b] = pool;
-        pool += 3 * align_up(B.reg_elems * esz);
+        pool += 3 * align_up(B.reg_elems * esz) + (e->mixed ? align_up(resid_bytes(nf, B.p.fsize)) : 0);
         met_off[b] = met;
         met += align_up(metric_len(B.p) * sizeof(double)) / sizeof(double);
         bg_off[b] = nzsum;
@@ -212,6 +213,7 @@ extern "C" int mhd_ens_create(const mhd_desc *descs, int nbox, mhd_ens_t **out) 
             D.h[0] = reinterpret_cast<float *>(base);
             D.h[1] = reinterpret_cast<float *>(base + rb);
             D.dr = reinterpret_cast<float *>(base + 2 * rb);
+            D.rr = reinterpret_cast<unsigned *>(base + 3 * rb);
             const size_t o = bg_off[b];
             // 4 FP64 and 5 FP32 tables of npz+2 entries, laid out per box
             D.bg = background_view(B.p.npz, e->bgd + 4 * o, e->bgf + 5 * o);
@@ -257,6 +259,10 @@ extern "C" int mhd_ens_set_metrics(mhd_ens_t *e, int b, const double *dxxdx, con
         EHIP(e, hipMemcpyAsync(const_cast<double *>(dst[n]), src[n], len[n] * sizeof(double), hipMemcpyHostToDevice,
                                e->stream));
     }
+    // the grid-only CFL term, which the stage kernels take from here
+    const double d2 = dd2_min(B.p, ddx, ddy, ddz);
+    EHIP(e, hipMemcpyAsync(reinterpret_cast<char *>(e->sc + b) + offsetof(DevScalars, dd2min), &d2, sizeof(d2),
+                           hipMemcpyHostToDevice, e->stream));
     const size_t nz = (size_t)B.p.npz + 2;
     B.dzzdz.assign(dzzdz, dzzdz + nz);
     B.d2zzdz2.assign(d2zzdz2, d2zzdz2 + nz);
@@ -310,8 +316,8 @@ extern "C" int mhd_ens_upload(mhd_ens_t *e, int b, const double *ru, const doubl
             EHIP(e, launch_pack(B.p, e->stage, D.q[0] + off, e->stream));
             EHIP(e, launch_pack(B.p, e->stage, D.q[1] + off, e->stream));
         } else {
-            EHIP(e, launch_pack_mixed(B.p, e->stage, f, D.bg, D.h[1], D.dr, e->stream));
-            EHIP(e, launch_pack_mixed(B.p, e->stage, f, D.bg, D.h[0], D.dr, e->stream));
+            EHIP(e, launch_pack_mixed(B.p, e->stage, f, D.bg, D.h[1], D.rr, e->stream));
+            EHIP(e, launch_pack_mixed(B.p, e->stage, f, D.bg, D.h[0], D.rr, e->stream));
         }
     }
     EHIP(e, hipStreamSynchronize(e->stream));
@@ -335,7 +341,7 @@ extern "C" int mhd_ens_download(mhd_ens_t *e, int b, double *ru, double *rv, dou
         if (!dst[f]) continue;
         const size_t off = (size_t)f * B.p.fsize;
         if (!e->mixed) EHIP(e, launch_unpack(B.p, D.q[e->cur] + off, e->stage, 1, e->stream));
-        else EHIP(e, launch_unpack_mixed(B.p, D.h[e->cur], D.dr, f, D.bg, e->stage, e->stream));
+        else EHIP(e, launch_unpack_mixed(B.p, D.h[e->cur], D.rr, f, D.bg, e->stage, e->stream));
         EHIP(e, hipMemcpyAsync(dst[f], e->stage, pbytes, hipMemcpyDeviceToHost, e->stream));
         EHIP(e, hipStreamSynchronize(e->stream));
     }
@@ -415,7 +421,7 @@ static int prime_cfl(mhd_ens_t *e) {
         if (B.cfl_valid) continue;
         const BoxDev &D = e->hbox[b];
         if (!e->mixed) EHIP(e, launch_cfl(B.p, D.m, D.q[e->cur], D.sc, e->stream));
-        else EHIP(e, launch_cfl_mixed(B.p, D.m, D.bg, D.h[e->cur], D.dr, D.sc, e->stream));
+        else EHIP(e, launch_cfl_mixed(B.p, D.m, D.bg, D.h[e->cur], D.rr, D.sc, e->stream));
         EHIP(e, launch_cfl_final(B.p, D.sc, false, e->stream));
         B.cfl_valid = true;
     }
@@ -565,11 +571,12 @@ extern "C" int mhd_ens_autotune(mhd_ens_t *e, int nsteps, char *report, int repo
     int rc = prime_cfl(e);
     if (rc) return rc;
     // save what the next step reads: the current register of every box (and
-    // D for the mixed form) and the device scalars; Z and the other
-    // register are rewritten by the first stage
+    // the residual register R for the mixed form) and the device scalars; Z
+    // or D and the other register are rewritten by the first stage
     const size_t esz = e->mixed ? sizeof(float) : sizeof(double);
+    const int nf = e->lmag ? 8 : 5;
     size_t tot = 0;
-    for (auto &B : e->box) tot += (e->mixed ? 2 : 1) * B.reg_elems * esz;
+    for (auto &B : e->box) tot += B.reg_elems * esz + (e->mixed ? resid_bytes(nf, B.p.fsize) : 0);
     char *save = nullptr;
     DevScalars *ssave = nullptr;
     EHIP(e, hipMalloc(&save, tot));
@@ -584,12 +591,12 @@ extern "C" int mhd_ens_autotune(mhd_ens_t *e, int nsteps, char *report, int repo
         hipError_t x = hipSuccess;
         for (int b = 0; b < e->nbox && x == hipSuccess; ++b) {
             const BoxDev &D = e->hbox[b];
-            const size_t nb = e->box[b].reg_elems * esz;
-            void *regs[2] = {e->mixed ? (void *)D.h[cur0] : (void *)D.q[cur0], (void *)D.dr};
+            const size_t nbs[2] = {e->box[b].reg_elems * esz, e->mixed ? resid_bytes(nf, e->box[b].p.fsize) : 0};
+            void *regs[2] = {e->mixed ? (void *)D.h[cur0] : (void *)D.q[cur0], (void *)D.rr};
             for (int i = 0; i < (e->mixed ? 2 : 1) && x == hipSuccess; ++i) {
-                x = to_save ? hipMemcpyAsync(save + off, regs[i], nb, hipMemcpyDeviceToDevice, e->stream)
-                            : hipMemcpyAsync(regs[i], save + off, nb, hipMemcpyDeviceToDevice, e->stream);
-                off += nb;
+                x = to_save ? hipMemcpyAsync(save + off, regs[i], nbs[i], hipMemcpyDeviceToDevice, e->stream)
+                            : hipMemcpyAsync(regs[i], save + off, nbs[i], hipMemcpyDeviceToDevice, e->stream);
+                off += nbs[i];
             }
         }
         if (x == hipSuccess)

@@ -69,7 +69,19 @@ inline DevScalars initial_scalars() {
     DevScalars s{};
     s.key_mach = dkey(0.0);
     for (int n = 0; n < 4; ++n) s.key_min[n] = dkey(INFINITY);
+    s.dd2min = INFINITY;
     return s;
+}
+
+// min over the points i = 1..npx, j = 1..npy, k = 1..npz of
+// min(DDX_i, DDY_j, DDZ_k)^2, the grid-only term of the magnetic CFL limit
+// (the same squares and minimum the per-point reduction forms).
+inline double dd2_min(const Params &p, const double *ddx, const double *ddy, const double *ddz) {
+    double m = INFINITY;
+    for (int i = 1; i <= p.npx; ++i) m = std::fmin(m, ddx[i] * ddx[i]);
+    for (int j = 1; j <= p.npy; ++j) m = std::fmin(m, ddy[j] * ddy[j]);
+    for (int k = 1; k <= p.npz; ++k) m = std::fmin(m, ddz[k] * ddz[k]);
+    return m;
 }
 
 // Number of doubles of the 11 metric arrays of a box.

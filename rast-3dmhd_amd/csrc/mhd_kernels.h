@@ -51,6 +51,7 @@ struct DevScalars {
     unsigned long long key_mach;      // max of OGAMMA*UU/TT
     unsigned long long key_min[4];    // mins of the CFL limits
     double fixed_dt;                  // > 0: step with this dt instead of the CFL dt
+    double dd2min;                    // min over all points of min(DDX,DDY,DDZ)^2 (metric only)
     int nit;
     int nonfinite;
 };
@@ -70,7 +71,7 @@ struct Background {
 };
 
 // One box of an ensemble as the batched kernels see it (device memory).
-// FP64 boxes use q/z, mixed boxes h/dr and bg.
+// FP64 boxes use q/z, mixed boxes h/dr/rr and bg.
 struct BoxDev {
     Params p;
     Metrics m;
@@ -79,8 +80,12 @@ struct BoxDev {
     double *z;
     float *h[2];
     float *dr;
+    unsigned *rr;
     DevScalars *sc;
 };
+
+// Bytes of the mixed form's residual register R for nf fields.
+inline size_t resid_bytes(int nf, long long fsize) { return (size_t)((nf + 1) / 2) * (size_t)fsize * 4; }
 
 // Work item of a batched stage launch: one tile column chunk of one box.
 struct StageItem {
@@ -118,7 +123,6 @@ hipError_t launch_stage_batch(int mode, int tile_y, int lmag, const BoxDev *boxe
 hipError_t launch_bcon_batch(int lmag, const BoxDev *boxes, const ColumnItem *items, int nitems, int a, bool cfl,
                              hipStream_t s);
 hipError_t launch_cfl_final_batch(const BoxDev *boxes, int nbox, bool clock, hipStream_t s);
-int stage_lds(int tile_y, int lmag, int mixed);
 hipError_t launch_box_filter(const Params &p, const double *q, int nf, int factor, int zmode, int nxc, int nyc, int nzc,
                              double *out, hipStream_t s);
 
@@ -127,25 +131,29 @@ hipError_t launch_box_filter(const Params &p, const double *q, int nf, int facto
 // per-plane means [8][npz], 1/std[8]) as BF16 [plane-1][j][i][8], and the
 // forcing from the network output (nrm_out = per-plane mean subgrid terms
 // [7][npz], std[7]); force_on is cleared if any output is not finite.
-hipError_t launch_closure_inputs(const Params &p, const Background &bg, const float *h, const float *dr,
+hipError_t launch_closure_inputs(const Params &p, const Background &bg, const float *h, const unsigned *rr,
                                  const float *nrm, __bf16 *x, hipStream_t s);
 hipError_t launch_closure_forcing(const Params &p, const float *y, const float *nrm_out, float *force, int *force_on,
                                   hipStream_t s);
 
-// Mixed-precision form (mhd_mixed.hip).
+// Mixed-precision form (mhd_mixed.hip).  Registers: h (FP32 stored values,
+// ping-pong), dr (FP32: residual + zeta dt F between stages 1-2 and 2-3) and
+// rr (the rounding residual of the stored values between steps as BF16, two
+// fields per 32-bit word: (nf+1)/2 words per point, written by stage 3 and
+// the upload, read by stage 1).
 hipError_t launch_stage_mixed(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,
-                              float *hout, float *dr, DevScalars *sc, hipStream_t s);
+                              float *hout, float *dr, unsigned *rr, DevScalars *sc, hipStream_t s);
 hipError_t launch_bcon_mixed(const Params &p, const Metrics &m, const Background &bg, float *h, float *dr,
-                             DevScalars *sc, bool last, hipStream_t s);
+                             unsigned *rr, DevScalars *sc, bool last, hipStream_t s);
 hipError_t launch_cfl_mixed(const Params &p, const Metrics &m, const Background &bg, const float *h,
-                            const float *dr, DevScalars *sc, hipStream_t s);
+                            const unsigned *rr, DevScalars *sc, hipStream_t s);
 // Conversion of field `field` between a padded FP64 array and the stored
-// values / residuals of the registers hreg, dreg (base pointers, all
+// values / resting residuals of the registers hreg, rreg (base pointers, all
 // fields).  Pack rho (field 3) before T (field 4).
 hipError_t launch_pack_mixed(const Params &p, const double *padded, int field, const Background &bg, float *hreg,
-                             float *dreg, hipStream_t s);
-hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const float *dreg, int field, const Background &bg,
-                               double *padded, hipStream_t s);
+                             unsigned *rreg, hipStream_t s);
+hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const unsigned *rreg, int field,
+                               const Background &bg, double *padded, hipStream_t s);
 
 hipError_t launch_stage_batch_mixed(int mode, int tile_y, int lmag, const BoxDev *boxes, const StageItem *items,
                                     int nitems, int grid, int a, hipStream_t s);

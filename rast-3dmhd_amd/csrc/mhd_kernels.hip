@@ -237,7 +237,7 @@ __device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, co
         double zk[8];
         if (MODE >= 2 && valid) {
 #pragma unroll
-            for (int f = 0; f < NF; ++f) zk[f] = zr[f * fs + o];
+            for (int f = 0; f < NF; ++f) zk[f] = __builtin_nontemporal_load(&zr[f * fs + o]);  // read once
         }
         __syncthreads();
         if (valid) {
@@ -884,12 +884,6 @@ hipError_t launch_bcon_batch(int lmag, const BoxDev *boxes, const ColumnItem *it
 hipError_t launch_cfl_final_batch(const BoxDev *boxes, int nbox, bool clock, hipStream_t s) {
     hipLaunchKernelGGL(k_cfl_final_batch, dim3((nbox + 63) / 64), dim3(64), 0, s, boxes, nbox, clock ? 1 : 0);
     return hipGetLastError();
-}
-
-int stage_lds(int tile_y, int lmag, int mixed) {
-    const int nf = lmag ? 8 : 5;
-    const int sp = (TX + 2) * (tile_y + 2);
-    return (2 * nf + 9) * sp * (mixed ? 4 : 8);
 }
 
 hipError_t launch_bcon(const Params &p, const Metrics &m, const double *qin, double *qout, DevScalars *sc,

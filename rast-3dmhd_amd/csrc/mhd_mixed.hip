@@ -19,13 +19,18 @@
 // the temperature slot is T'(rho', P') as the stencil computes it (trep).  The second
 // register D (FP32) carries, per Wray's low-storage scheme, the zeta*dt*F
 // increment of the next stage plus the rounding residual of the stored
-// value (for T: X_T - T'(new rho', new P')), and after the last stage the
-// residual alone, so that (h, D) is a compensated pair and the storage
-// rounding does not accumulate from step to step.
+// value (for T: X_T - T'(new rho', new P')).  After the last stage only the
+// residual is left, at most a few FP32 ulps of the value: it is kept in a
+// third register R as BF16, two fields per 32-bit word (8 significant bits
+// of a residual of relative size 2^-24, i.e. the state to about 2^-33), so
+// that (h, R) is a
+// compensated pair and the storage rounding does not accumulate from step
+// to step.  Stage 1 reads R, stages 1-2 write D, stage 3 writes R.
 // The CFL reduction, the boundary planes and the background are FP64.
 //
-// Traffic: every stage reads h (stencil) and D and writes h and D:
-// 4 x 8 fields x 4 B = 128 B/zone/stage, 384 B/zone/step (FP64: 640).
+// Traffic per zone (8 fields): stage 1 reads h (4 B) and R (2 B) and writes
+// h and D (4 + 4 B); stage 2 reads and writes h and D; stage 3 reads h and D
+// and writes h and R: 112 + 128 + 112 = 352 B/zone/step (FP64: 640).
 
 #include "mhd_device.h"
 
@@ -47,10 +52,14 @@ struct MTile {
     static_assert(NHALO <= NT, "one halo point per thread at most");
 };
 
-// LDS: raw planes k, k+1 (2 x NF fields) and a three-plane ring of u, v, w, T'.
-template <int TY, bool MAG>
+// LDS: raw planes k, k+1 (2 x NF fields) and a three-plane ring of u, v, w,
+// T'; in stage 3 also each thread's running CFL values (4 doubles), which
+// would otherwise stay live in registers across the whole z march.
+constexpr int CFL_SLOTS = 4;
+template <int TY, bool MAG, int MODE>
 constexpr int mixed_lds_bytes() {
-    return (2 * (MAG ? 8 : 5) + 12) * MTile<TY>::SP * (int)sizeof(float);
+    return (2 * (MAG ? 8 : 5) + 12) * MTile<TY>::SP * (int)sizeof(float) +
+           (MODE == 3 ? CFL_SLOTS * MTX * TY * (int)sizeof(double) : 0);
 }
 
 // T' represented by a stored (rho', P') pair: (P' - T0 rho') / (rho0 + rho')
@@ -67,6 +76,30 @@ __host__ __device__ __forceinline__ float trep(float r0, float t0, float rh, flo
 // rounding: T0 times the rounding error of rho' would be comparable to P'.
 __host__ __device__ __forceinline__ double pprime(double r0, double t0, double rh, double tp) {
     return r0 * tp + t0 * rh + rh * tp;
+}
+
+// BF16 bit patterns of the resting residual R: round to nearest even
+// (NaN stays NaN), and back (exact).
+__host__ __device__ __forceinline__ unsigned short to_bf16(float x) {
+    unsigned u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__host__ __device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// R packs the BF16 residuals of two fields per 32-bit word: word
+// (f/2)*fs + o holds field f in its low (f even) or high (f odd) half.
+__device__ __forceinline__ float r_half(unsigned w, int f) {
+    return from_bf16((unsigned short)((f & 1) ? (w >> 16) : (w & 0xffffu)));
+}
+__device__ __forceinline__ float r_get(const unsigned *__restrict__ rr, long long fs, int f, long long o) {
+    return r_half(rr[(f >> 1) * fs + o], f);
+}
+__device__ __forceinline__ void r_put(unsigned *__restrict__ rr, long long fs, int f, long long o, float r) {
+    unsigned *const w = &rr[(f >> 1) * fs + o];
+    const unsigned b = to_bf16(r);
+    *w = (f & 1) ? ((*w & 0x0000ffffu) | (b << 16)) : ((*w & 0xffff0000u) | b);
 }
 
 // CFL limits of the mixed form (FP64), accumulated in monotone transformed
@@ -113,7 +146,9 @@ __device__ __forceinline__ DdPair ddpair(double ddx, double ddy, double ddz) {
     return DdPair{d2, 1.0 / d2};
 }
 
-__device__ void cflf_reduce(const Params &p, CflFast c, DevScalars *__restrict__ sc) {
+// d2extra: a min(dd^2) contribution computed elsewhere (the grid-only term
+// of the stage kernel, which does not reduce it per point).
+__device__ void cflf_reduce(const Params &p, CflFast c, DevScalars *__restrict__ sc, double d2extra = INFINITY) {
     __shared__ double red[16][5];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -136,6 +171,7 @@ __device__ void cflf_reduce(const Params &p, CflFast c, DevScalars *__restrict__
             c.s3 = nmin(c.s3, red[v][3]);
             c.d2 = nmin(c.d2, red[v][4]);
         }
+        c.d2 = nmin(c.d2, d2extra);
         const double w[4] = {1.0 / sqrt(c.q1), 0.5 * p.repr * p.cv * c.s2, 0.375 * p.re * c.s3, 0.5 * p.rm * c.d2};
         atomicMax(&sc->key_mach, dkey(c.mach));
         for (int n = 0; n < 4; ++n) {
@@ -161,13 +197,16 @@ __device__ __forceinline__ void load_rawf(const float *__restrict__ q, long long
 template <int TY, bool MAG, int MODE>
 __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, const Background &bg,
                                              const float *__restrict__ hin, float *__restrict__ hout,
-                                             float *__restrict__ dr, DevScalars *__restrict__ sc, int i0, int j0,
-                                             int kbeg, int kend, float *ldsf) {
+                                             float *__restrict__ dr, unsigned *__restrict__ rr,
+                                             DevScalars *__restrict__ sc, int i0, int j0, int kbeg, int kend,
+                                             float *ldsf) {
     using T = MTile<TY>;
     constexpr int SW = T::SW, SP = T::SP;
     constexpr int NF = MAG ? 8 : 5;
     float *const s_cur = ldsf;
     float *const s_ring = ldsf + 2 * NF * SP;
+    // stage 3: running CFL maxima/minima of this thread, [slot][thread]
+    double *const s_cfl = reinterpret_cast<double *>(ldsf + (2 * NF + 12) * SP);
 
     const int t = threadIdx.x;
     const int npx = p.npx, npy = p.npy, npz = p.npz;
@@ -237,8 +276,14 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
     const double dt = sc->dt;
     const double coefg = (MODE == 1 ? p.gam1 : MODE == 2 ? p.gam2 : p.gam3) * dt;
     const double coefz = (MODE == 1 ? p.zeta1 : p.zeta2) * dt;
-    CflFast cfl = cflf_init();
     const double c_ddx = MODE == 3 ? m.ddx[gix] : 0.0, c_ddy = MODE == 3 ? m.ddy[gjy] : 0.0;
+    if (MODE == 3) {
+        const CflFast c0 = cflf_init();
+        s_cfl[0 * T::NT + t] = c0.mach;
+        s_cfl[1 * T::NT + t] = c0.q1;
+        s_cfl[2 * T::NT + t] = c0.s2;
+        s_cfl[3 * T::NT + t] = c0.s3;
+    }
     const bool frc = bg.force != nullptr && *bg.force_on != 0;
 
 #pragma unroll 1
@@ -252,7 +297,14 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         float dk_[8];
         if (valid) {
 #pragma unroll
-            for (int f = 0; f < NF; ++f) dk_[f] = __builtin_nontemporal_load(&dr[f * fs + o]);  // read once
+            for (int f = 0; f < NF; ++f) {  // read once
+                if (MODE != 1) dk_[f] = __builtin_nontemporal_load(&dr[f * fs + o]);
+                else if (!(f & 1)) {
+                    const unsigned w = __builtin_nontemporal_load(&rr[(f >> 1) * fs + o]);
+                    dk_[f] = r_half(w, f);
+                    if (f + 1 < NF) dk_[f + 1] = r_half(w, f + 1);
+                }
+            }
         }
         __syncthreads();
         if (valid) {
@@ -439,12 +491,18 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 if (f != 4) hn[f] = (float)xq[f];
             }
             hn[4] = (float)pprime(r0d, t0d, xq[3], xq[4]);
+            unsigned rw = 0;
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const double r = xq[f] - (f == 4 ? (double)trep(bg.ro32[k], bg.tt32[k], hn[3], hn[4]) : (double)hn[f]);
                 __builtin_nontemporal_store(hn[f], &hout[f * fs + o]);
-                const float dn = (MODE < 3) ? (float)(r + coefz * Fd[f]) : (float)r;
-                __builtin_nontemporal_store(dn, &dr[f * fs + o]);
+                if (MODE < 3) {
+                    __builtin_nontemporal_store((float)(r + coefz * Fd[f]), &dr[f * fs + o]);
+                } else {
+                    const unsigned b = to_bf16((float)r);
+                    rw = (f & 1) ? (rw | (b << 16)) : b;
+                    if ((f & 1) || f + 1 == NF) __builtin_nontemporal_store(rw, &rr[(f >> 1) * fs + o]);
+                }
             }
             if (MODE == 3 && !top && !bot) {
                 double full[8];
@@ -453,7 +511,13 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 full[3] += bg.ro[k];
                 full[4] += bg.tt[k];
                 const DdPair d = ddpair(c_ddx, c_ddy, m.ddz[k]);
-                cflf_point<MAG>(p, full, d.dd2, d.idd2, 1.0 / m.rkapa[k], cfl);
+                CflFast c{s_cfl[0 * T::NT + t], s_cfl[1 * T::NT + t], s_cfl[2 * T::NT + t], s_cfl[3 * T::NT + t],
+                          INFINITY};
+                cflf_point<MAG>(p, full, d.dd2, d.idd2, 1.0 / m.rkapa[k], c);
+                s_cfl[0 * T::NT + t] = c.mach;
+                s_cfl[1 * T::NT + t] = c.q1;
+                s_cfl[2 * T::NT + t] = c.s2;
+                s_cfl[3 * T::NT + t] = c.s3;
             }
 #pragma unroll
             for (int f = 0; f < NF; ++f) rm_.v[f] = h0[f];
@@ -468,14 +532,20 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             }
         }
     }
-    if (MODE == 3) cflf_reduce(p, cfl, sc);
+    if (MODE == 3) {
+        // min(dd^2) depends on the grid only: its minimum over all points
+        // comes with the metrics
+        const CflFast c{s_cfl[0 * T::NT + t], s_cfl[1 * T::NT + t], s_cfl[2 * T::NT + t], s_cfl[3 * T::NT + t],
+                        INFINITY};
+        cflf_reduce(p, c, sc, MAG ? sc->dd2min : INFINITY);
+    }
 }
 
 template <int TY, bool MAG, int MODE>
 __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metrics m, const Background bg,
                                                      const float *__restrict__ hin, float *__restrict__ hout,
-                                                     float *__restrict__ dr, DevScalars *__restrict__ sc,
-                                                     int kchunk) {
+                                                     float *__restrict__ dr, unsigned *__restrict__ rr,
+                                                     DevScalars *__restrict__ sc, int kchunk) {
     extern __shared__ float ldsf[];
     const int ntx = (p.npx + MTX - 1) / MTX, nty = (p.npy + TY - 1) / TY;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -485,7 +555,7 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
     const int kbeg = 1 + tz * kchunk;
     const int kend = min(p.npz, kbeg + kchunk - 1);
     if (kbeg > p.npz) return;
-    stage_tile_m<TY, MAG, MODE>(p, m, bg, hin, hout, dr, sc, txb * MTX, tyb * TY, kbeg, kend, ldsf);
+    stage_tile_m<TY, MAG, MODE>(p, m, bg, hin, hout, dr, rr, sc, txb * MTX, tyb * TY, kbeg, kend, ldsf);
 }
 
 // Ensemble: persistent grid over the work items of all boxes (static stride).
@@ -498,19 +568,21 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m_batch(const BoxDev *__restr
         const StageItem it = items[w];
         const BoxDev &B = boxes[it.box];
         __syncthreads();  // the previous item's last LDS reads are done
-        stage_tile_m<TY, MAG, MODE>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.sc, it.i0, it.j0, it.kbeg, it.kend,
-                                    ldsf);
+        stage_tile_m<TY, MAG, MODE>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.rr, B.sc, it.i0, it.j0, it.kbeg,
+                                    it.kend, ldsf);
     }
 }
 
-// BCON of the mixed form, in FP64 on the full state bg + value (+ D after
-// the last stage, where D holds only the rounding residual).  New boundary
-// values are split again into stored values and the residual D; a new
-// boundary temperature is stored as the P' of the plane's stored rho'.
+// BCON of the mixed form, in FP64 on the full state bg + value (+ R after
+// the last stage, where R holds the rounding residual).  New boundary
+// values are split again into stored values and the residual (D after
+// stages 1-2, R after stage 3); a new boundary temperature is stored as the
+// P' of the plane's stored rho'.
 template <bool MAG, bool LAST>
 __device__ __forceinline__ void bcon_column_m(const Params &p, const Metrics &m, const Background &bg,
                                               float *__restrict__ h, float *__restrict__ dr,
-                                              DevScalars *__restrict__ sc, int idx) {
+                                              unsigned *__restrict__ rr, DevScalars *__restrict__ sc,
+                                              int idx) {
     CflFast cfl = cflf_init();
     if (idx < p.npx * p.npy) {
         const long long fs = p.fsize, pl = p.plane;
@@ -519,26 +591,30 @@ __device__ __forceinline__ void bcon_column_m(const Params &p, const Metrics &m,
         auto full = [&](int f, int k) -> double {
             const long long o = f * fs + (long long)k * pl + idx;
             double v = f == 4 ? (double)trep(bg.ro32[k], bg.tt32[k], h[o - fs], h[o]) : (double)h[o];
-            if (LAST) v += (double)dr[o];
+            if (LAST) v += (double)r_get(rr, fs, f, (long long)k * pl + idx);
             if (f == 3) v += bg.ro[k];
             if (f == 4) v += bg.tt[k];
             return v;
+        };
+        auto residual = [&](int f, int k, double r) {
+            if (LAST) r_put(rr, fs, f, (long long)k * pl + idx, (float)r);
+            else dr[f * fs + (long long)k * pl + idx] = (float)r;
         };
         auto put = [&](int f, int k, double v) {
             const long long o = f * fs + (long long)k * pl + idx;
             if (f == 4) {
                 const double tp = v - bg.tt[k];
                 const float rh = h[o - fs];
-                const double rhx = LAST ? (double)rh + (double)dr[o - fs] : (double)rh;
+                const double rhx = LAST ? (double)rh + (double)r_get(rr, fs, 3, (long long)k * pl + idx) : (double)rh;
                 const float hp = (float)pprime(bg.ro[k], bg.tt[k], rhx, tp);
                 h[o] = hp;
-                dr[o] = (float)(tp - (double)trep(bg.ro32[k], bg.tt32[k], rh, hp));
+                residual(4, k, tp - (double)trep(bg.ro32[k], bg.tt32[k], rh, hp));
                 return;
             }
             if (f == 3) v -= bg.ro[k];
             const float hn = (float)v;
             h[o] = hn;
-            dr[o] = (float)(v - (double)hn);
+            residual(f, k, v - (double)hn);
         };
         for (int side = 0; side < 2; ++side) {
             const int a = side ? nz : 1, b = side ? nz - 1 : 2, c = side ? nz - 2 : 3;
@@ -575,8 +651,8 @@ __device__ __forceinline__ void bcon_column_m(const Params &p, const Metrics &m,
 template <bool MAG, bool LAST>
 __global__ __launch_bounds__(256) void k_bcon_m(const Params p, const Metrics m, const Background bg,
                                                 float *__restrict__ h, float *__restrict__ dr,
-                                                DevScalars *__restrict__ sc) {
-    bcon_column_m<MAG, LAST>(p, m, bg, h, dr, sc, blockIdx.x * blockDim.x + threadIdx.x);
+                                                unsigned *__restrict__ rr, DevScalars *__restrict__ sc) {
+    bcon_column_m<MAG, LAST>(p, m, bg, h, dr, rr, sc, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Ensemble: one workgroup per (box, run of 256 columns) of register 1-a.
@@ -585,13 +661,13 @@ __global__ __launch_bounds__(256) void k_bcon_m_batch(const BoxDev *__restrict__
                                                       const ColumnItem *__restrict__ items, int a) {
     const ColumnItem it = items[blockIdx.x];
     const BoxDev &B = boxes[it.box];
-    bcon_column_m<MAG, LAST>(B.p, B.m, B.bg, B.h[1 - a], B.dr, B.sc, it.col0 + threadIdx.x);
+    bcon_column_m<MAG, LAST>(B.p, B.m, B.bg, B.h[1 - a], B.dr, B.rr, B.sc, it.col0 + threadIdx.x);
 }
 
 // Stand-alone CFL of the mixed state (first step after an upload).
 template <bool MAG>
 __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, const Background bg,
-                                               const float *__restrict__ h, const float *__restrict__ dr,
+                                               const float *__restrict__ h, const unsigned *__restrict__ rr,
                                                DevScalars *__restrict__ sc) {
     const long long fs = p.fsize, pl = p.plane;
     const long long n = (long long)p.npz * pl;
@@ -605,7 +681,7 @@ __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, 
         double v[8];
         for (int f = 0; f < (MAG ? 8 : 5); ++f) {
             const double x = f == 4 ? (double)trep(bg.ro32[k], bg.tt32[k], h[3 * fs + o], h[4 * fs + o]) : (double)h[f * fs + o];
-            v[f] = x + (double)dr[f * fs + o];
+            v[f] = x + (double)r_get(rr, fs, f, o);
         }
         v[3] += bg.ro[k];
         v[4] += bg.tt[k];
@@ -615,15 +691,14 @@ __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, 
     cflf_reduce(p, cfl, sc);
 }
 
-// padded FP64 field f -> (stored value, D = rounding residual) in the
-// registers hreg, dreg (all fields); the temperature is stored as the P' of
+// padded FP64 field f -> (stored value, R = rounding residual) in the
+// registers hreg, rreg (all fields); the temperature is stored as the P' of
 // the already packed rho' of the same register.
 __global__ void k_pack_m(const Params p, const double *__restrict__ padded, int field, const Background bg,
-                         float *__restrict__ hreg, float *__restrict__ dreg) {
+                         float *__restrict__ hreg, unsigned *__restrict__ rreg) {
     const long long n = p.fsize, fs = p.fsize;
     const int nxp = p.npx + 2, nyp = p.npy + 2;
     float *const h = hreg + field * fs;
-    float *const dr = dreg + field * fs;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long long)gridDim.x * blockDim.x) {
         const long long k = idx / p.plane;
@@ -633,21 +708,21 @@ __global__ void k_pack_m(const Params p, const double *__restrict__ padded, int 
         if (field == 4) {
             const double tp = v - bg.tt[k];
             const float rh = hreg[3 * fs + idx];
-            const float hp = (float)pprime(bg.ro[k], bg.tt[k], (double)rh + (double)dreg[3 * fs + idx], tp);
+            const float hp = (float)pprime(bg.ro[k], bg.tt[k], (double)rh + (double)r_get(rreg, fs, 3, idx), tp);
             h[idx] = hp;
-            dr[idx] = (float)(tp - (double)trep(bg.ro32[k], bg.tt32[k], rh, hp));
+            r_put(rreg, fs, 4, idx, (float)(tp - (double)trep(bg.ro32[k], bg.tt32[k], rh, hp)));
             continue;
         }
         if (field == 3) v -= bg.ro[k];
         const float hn = (float)v;
         h[idx] = hn;
-        dr[idx] = (float)(v - (double)hn);
+        r_put(rreg, fs, field, idx, (float)(v - (double)hn));
     }
 }
 
-// field f of (stored value, D) -> padded FP64 field with periodic ghosts
-__global__ void k_unpack_m(const Params p, const float *__restrict__ hreg, const float *__restrict__ dreg, int field,
-                           const Background bg, double *__restrict__ padded) {
+// field f of (stored value, R) -> padded FP64 field with periodic ghosts
+__global__ void k_unpack_m(const Params p, const float *__restrict__ hreg, const unsigned *__restrict__ rreg,
+                           int field, const Background bg, double *__restrict__ padded) {
     const int nxp = p.npx + 2, nyp = p.npy + 2, nzp = p.npz + 2;
     const long long n = (long long)nxp * nyp * nzp, fs = p.fsize;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
@@ -659,22 +734,23 @@ __global__ void k_unpack_m(const Params p, const float *__restrict__ hreg, const
         double v;
         if (field == 4) v = bg.tt[k] + (double)trep(bg.ro32[k], bg.tt32[k], hreg[3 * fs + o], hreg[4 * fs + o]);
         else v = (double)hreg[field * fs + o] + (field == 3 ? bg.ro[k] : 0.0);
-        padded[idx] = v + (double)dreg[field * fs + o];
+        padded[idx] = v + (double)r_get(rreg, fs, field, o);
     }
 }
 
-// Network inputs of the closure from the state (h, D) about the background.
+// Network inputs of the closure from the state (h, R) about the background.
 __global__ void k_closure_inputs(const Params p, const Background bg, const float *__restrict__ h,
-                                 const float *__restrict__ dr, const float *__restrict__ nrm, __bf16 *__restrict__ x) {
+                                 const unsigned *__restrict__ rr, const float *__restrict__ nrm,
+                                 __bf16 *__restrict__ x) {
     const long long fs = p.fsize, n = (long long)p.npz * p.plane;
     for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
          v += (long long)gridDim.x * blockDim.x) {
         const int k = (int)(v / p.plane) + 1;
         const long long o = p.plane + v;
-        auto val = [&](int f) { return (double)h[f * fs + o] + (double)dr[f * fs + o]; };
+        auto val = [&](int f) { return (double)h[f * fs + o] + (double)r_get(rr, fs, f, o); };
         const double ro = bg.ro[k] + val(3);
         const double tt = bg.tt[k] + (double)trep(bg.ro32[k], bg.tt32[k], h[3 * fs + o], h[4 * fs + o]) +
-                          (double)dr[4 * fs + o];
+                          (double)r_get(rr, fs, 4, o);
         const double ri = 1.0 / ro;
         double c[8];
         c[0] = ro;
@@ -726,8 +802,8 @@ static int chunk_m(const Params &p, int ty, int *nchunks) {
 
 template <int TY, bool MAG, int MODE>
 static hipError_t launch_m(const Params &p, const Metrics &m, const Background &bg, const float *hin, float *hout,
-                           float *dr, DevScalars *sc, hipStream_t s) {
-    constexpr int lds = mixed_lds_bytes<TY, MAG>();
+                           float *dr, unsigned *rr, DevScalars *sc, hipStream_t s) {
+    constexpr int lds = mixed_lds_bytes<TY, MAG, MODE>();
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_m<TY, MAG, MODE>),
@@ -739,74 +815,74 @@ static hipError_t launch_m(const Params &p, const Metrics &m, const Background &
     const int kc = chunk_m(p, TY, &nch);
     const int ntiles = ((p.npx + MTX - 1) / MTX) * ((p.npy + TY - 1) / TY);
     hipLaunchKernelGGL((k_stage_m<TY, MAG, MODE>), dim3(ntiles * nch), dim3(MTX * TY), lds, s, p, m, bg, hin, hout,
-                       dr, sc, kc);
+                       dr, rr, sc, kc);
     return hipGetLastError();
 }
 
 template <int TY>
 static hipError_t launch_m_ty(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,
-                              float *hout, float *dr, DevScalars *sc, hipStream_t s) {
+                              float *hout, float *dr, unsigned *rr, DevScalars *sc, hipStream_t s) {
     if (p.lmag) {
-        if (mode == 1) return launch_m<TY, true, 1>(p, m, bg, hin, hout, dr, sc, s);
-        if (mode == 2) return launch_m<TY, true, 2>(p, m, bg, hin, hout, dr, sc, s);
-        return launch_m<TY, true, 3>(p, m, bg, hin, hout, dr, sc, s);
+        if (mode == 1) return launch_m<TY, true, 1>(p, m, bg, hin, hout, dr, rr, sc, s);
+        if (mode == 2) return launch_m<TY, true, 2>(p, m, bg, hin, hout, dr, rr, sc, s);
+        return launch_m<TY, true, 3>(p, m, bg, hin, hout, dr, rr, sc, s);
     }
-    if (mode == 1) return launch_m<TY, false, 1>(p, m, bg, hin, hout, dr, sc, s);
-    if (mode == 2) return launch_m<TY, false, 2>(p, m, bg, hin, hout, dr, sc, s);
-    return launch_m<TY, false, 3>(p, m, bg, hin, hout, dr, sc, s);
+    if (mode == 1) return launch_m<TY, false, 1>(p, m, bg, hin, hout, dr, rr, sc, s);
+    if (mode == 2) return launch_m<TY, false, 2>(p, m, bg, hin, hout, dr, rr, sc, s);
+    return launch_m<TY, false, 3>(p, m, bg, hin, hout, dr, rr, sc, s);
 }
 
 hipError_t launch_stage_mixed(int mode, const Params &p, const Metrics &m, const Background &bg, const float *hin,
-                              float *hout, float *dr, DevScalars *sc, hipStream_t s) {
-    if (p.tile_y == 8) return launch_m_ty<8>(mode, p, m, bg, hin, hout, dr, sc, s);
-    return launch_m_ty<4>(mode, p, m, bg, hin, hout, dr, sc, s);
+                              float *hout, float *dr, unsigned *rr, DevScalars *sc, hipStream_t s) {
+    if (p.tile_y == 8) return launch_m_ty<8>(mode, p, m, bg, hin, hout, dr, rr, sc, s);
+    return launch_m_ty<4>(mode, p, m, bg, hin, hout, dr, rr, sc, s);
 }
 
 hipError_t launch_bcon_mixed(const Params &p, const Metrics &m, const Background &bg, float *h, float *dr,
-                             DevScalars *sc, bool last, hipStream_t s) {
+                             unsigned *rr, DevScalars *sc, bool last, hipStream_t s) {
     const int n = p.npx * p.npy;
     dim3 grid((n + 255) / 256), block(256);
     if (p.lmag) {
-        if (last) hipLaunchKernelGGL((k_bcon_m<true, true>), grid, block, 0, s, p, m, bg, h, dr, sc);
-        else hipLaunchKernelGGL((k_bcon_m<true, false>), grid, block, 0, s, p, m, bg, h, dr, sc);
+        if (last) hipLaunchKernelGGL((k_bcon_m<true, true>), grid, block, 0, s, p, m, bg, h, dr, rr, sc);
+        else hipLaunchKernelGGL((k_bcon_m<true, false>), grid, block, 0, s, p, m, bg, h, dr, rr, sc);
     } else {
-        if (last) hipLaunchKernelGGL((k_bcon_m<false, true>), grid, block, 0, s, p, m, bg, h, dr, sc);
-        else hipLaunchKernelGGL((k_bcon_m<false, false>), grid, block, 0, s, p, m, bg, h, dr, sc);
+        if (last) hipLaunchKernelGGL((k_bcon_m<false, true>), grid, block, 0, s, p, m, bg, h, dr, rr, sc);
+        else hipLaunchKernelGGL((k_bcon_m<false, false>), grid, block, 0, s, p, m, bg, h, dr, rr, sc);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_cfl_mixed(const Params &p, const Metrics &m, const Background &bg, const float *h,
-                            const float *dr, DevScalars *sc, hipStream_t s) {
+                            const unsigned *rr, DevScalars *sc, hipStream_t s) {
     const long long n = (long long)p.npz * p.plane;
     long long nb = (n + 255) / 256;
     if (nb > 4096) nb = 4096;
-    if (p.lmag) hipLaunchKernelGGL(k_cfl_m<true>, dim3((unsigned)nb), dim3(256), 0, s, p, m, bg, h, dr, sc);
-    else hipLaunchKernelGGL(k_cfl_m<false>, dim3((unsigned)nb), dim3(256), 0, s, p, m, bg, h, dr, sc);
+    if (p.lmag) hipLaunchKernelGGL(k_cfl_m<true>, dim3((unsigned)nb), dim3(256), 0, s, p, m, bg, h, rr, sc);
+    else hipLaunchKernelGGL(k_cfl_m<false>, dim3((unsigned)nb), dim3(256), 0, s, p, m, bg, h, rr, sc);
     return hipGetLastError();
 }
 
 hipError_t launch_pack_mixed(const Params &p, const double *padded, int field, const Background &bg, float *hreg,
-                             float *dreg, hipStream_t s) {
+                             unsigned *rreg, hipStream_t s) {
     long long nb = (p.fsize + 255) / 256;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(k_pack_m, dim3((unsigned)nb), dim3(256), 0, s, p, padded, field, bg, hreg, dreg);
+    hipLaunchKernelGGL(k_pack_m, dim3((unsigned)nb), dim3(256), 0, s, p, padded, field, bg, hreg, rreg);
     return hipGetLastError();
 }
 
-hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const float *dreg, int field, const Background &bg,
-                               double *padded, hipStream_t s) {
+hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const unsigned *rreg, int field,
+                               const Background &bg, double *padded, hipStream_t s) {
     const long long n = (long long)(p.npx + 2) * (p.npy + 2) * (p.npz + 2);
     long long nb = (n + 255) / 256;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(k_unpack_m, dim3((unsigned)nb), dim3(256), 0, s, p, hreg, dreg, field, bg, padded);
+    hipLaunchKernelGGL(k_unpack_m, dim3((unsigned)nb), dim3(256), 0, s, p, hreg, rreg, field, bg, padded);
     return hipGetLastError();
 }
 
 template <int TY, bool MAG, int MODE>
 static hipError_t launch_mb(const BoxDev *boxes, const StageItem *items, int nitems, int grid, int a,
                             hipStream_t s) {
-    constexpr int lds = mixed_lds_bytes<TY, MAG>();
+    constexpr int lds = mixed_lds_bytes<TY, MAG, MODE>();
     static bool attr = false;
     if (!attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_m_batch<TY, MAG, MODE>),
@@ -851,11 +927,11 @@ hipError_t launch_bcon_batch_mixed(int lmag, const BoxDev *boxes, const ColumnIt
     return hipGetLastError();
 }
 
-hipError_t launch_closure_inputs(const Params &p, const Background &bg, const float *h, const float *dr,
+hipError_t launch_closure_inputs(const Params &p, const Background &bg, const float *h, const unsigned *rr,
                                  const float *nrm, __bf16 *x, hipStream_t s) {
     long long nb = ((long long)p.npz * p.plane + 255) / 256;
     if (nb > 8192) nb = 8192;
-    hipLaunchKernelGGL(k_closure_inputs, dim3((unsigned)nb), dim3(256), 0, s, p, bg, h, dr, nrm, x);
+    hipLaunchKernelGGL(k_closure_inputs, dim3((unsigned)nb), dim3(256), 0, s, p, bg, h, rr, nrm, x);
     return hipGetLastError();
 }
 

@@ -72,7 +72,8 @@ C
       INTEGER(C_INT) :: IRC
       REAL(C_DOUBLE) :: DTC, UMC
       CHARACTER(LEN=16) :: PREC
-      INTEGER :: PLEN, PST
+      INTEGER :: PLEN, PST, IB, JB, KB
+      REAL(C_DOUBLE) :: RO0(NZ), TT0(NZ)
 C
       IF (.NOT.READY) THEN
          IF (NPEY*NPEZ.NE.1) THEN
@@ -141,6 +142,24 @@ C        RAST3DMHD_PRECISION=mixed selects the mixed-precision form
          IRC=MHD_SET_METRICS(H,DXXDX,D2XXDX2,DDX,DYYDY,D2YYDY2,DDY,
      2                       DZZDZ,D2ZZDZ2,DDZ,RKAPA,DKAPA)
          IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_set_metrics',IRC)
+         IF (D%PRECISION.EQ.MHD_MIXED) THEN
+C           background of the mixed form: horizontal means of rho and T
+            DO KB=1,NZ
+               RO0(KB)=0.0D0
+               TT0(KB)=0.0D0
+               DO JB=2,NY-1
+                  DO IB=2,NX-1
+                     RO0(KB)=RO0(KB)+RO(IB,JB,KB)
+                     TT0(KB)=TT0(KB)+TT(IB,JB,KB)
+                  ENDDO
+               ENDDO
+               RO0(KB)=RO0(KB)/DBLE((NX-2)*(NY-2))
+               TT0(KB)=TT0(KB)/DBLE((NX-2)*(NY-2))
+            ENDDO
+            IRC=MHD_SET_BACKGROUND(H,RO0,TT0)
+            IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_set_background',
+     2                                  IRC)
+         ENDIF
          IRC=MHD_UPLOAD(H,RU,RV,RW,RO,TT,BX,BY,BZ)
          IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_upload',IRC)
          IRC=MHD_SET_CLOCK(H,INT(NIT,C_INT),TIMC,TIMI)
