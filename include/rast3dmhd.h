@@ -137,6 +137,21 @@ int mhd_fluxes(mhd_t *h, double *fu, double *fv, double *fw, double *fr, double 
  * [field][K][J][I] with I fastest, interior points only.  MHD_FP64 only. */
 int mhd_box_filter(mhd_t *h, int factor, int zmode, int source, double *out);
 
+/* Horizontal means and mean flux profiles of the current state (a
+ * diagnostic, either precision), for the NPZ planes K=2..NPZ+1 (array
+ * element k-1 for plane k = K-1):
+ *   ttm, rom  HORIZONTAL_MEAN of TT and RO (3dmhdsub.f:3143-3223, the plain
+ *             mean over the NPX*NPY interior points);
+ *   fconm     mean convective flux, the horizontal mean of
+ *             -RW*(2.5*(TT-TTM) + 0.5*|RU,RV,RW|^2/RO^2) (3dmhdsub.f:1650-1661);
+ *   fradm     mean radiative flux (TTM(K+1)-TTM(K-1))*HZ*DZZDZ*RKAPA/REPR,
+ *             one-sided on the two boundary planes (3dmhdsub.f:1663-1677).
+ * The reference forms these only inside its flux-relaxation branch; here
+ * they can be taken at any time.  The sums run in a fixed parallel order
+ * (reproducible, equal to the reference's sequential sums to rounding).
+ * Any pointer may be NULL; needs NPZ >= 3. */
+int mhd_flux_profiles(mhd_t *h, double *ttm, double *rom, double *fconm, double *fradm);
+
 /* Measurement hook (no counterpart in the reference): runs n steps like
  * mhd_steps with HIP events around every launch on the handle's stream and
  * returns the mean duration in milliseconds of one step, one RK stage

@@ -801,3 +801,61 @@ void orc_step(orc_state *s)
     s->timc = s->timc + s->dt;
     s->timt = s->timi + s->timc;
 }
+
+/* -------------------------------------------------------------------------
+ * Horizontal means and mean flux profiles (a diagnostic, not part of STEP).
+ *
+ * HORIZONTAL_MEAN (sub:3143-3223, NGRID=0 branch, NPEY=1): for every plane
+ * K=1..NZ the sequential sum over J (outer) and I (inner) of the interior
+ * points, divided by FLOAT(NPX*NPY) (a float32 value, exact here).
+ * FCONM (sub:1650-1661): the horizontal mean of
+ *     WW2 = -RW*(2.5*WW1 + 0.5*(UU*UU+VV*VV+WW*WW)),
+ * WW1 = TT - TTM(K) (sub:1523-1531), UU = RU*(1/RO) as FLUXES forms it
+ * (sub:1390-1396).  FRADM (sub:1663-1677): central differences of TTM times
+ * HZ*DZZDZ*RKAPA/REPR, one-sided at the two boundary planes.
+ * Outputs have NZ entries (padded K=1..NZ, stored at K-1); FCONM and FRADM
+ * are filled for the planes K=2..NZ-1 only (the others are set to 0).
+ * ---------------------------------------------------------------------- */
+static double hmean_plane(const orc_state *s, const double *v, int k)
+{
+    const int nx = s->nx, ny = s->ny;
+    double w = 0.0;
+    for (int j = 2; j <= ny - 1; j++)
+        for (int i = 2; i <= nx - 1; i++)
+            w = w + A(v, i, j, k);
+    return w / (double)(float)((nx - 2) * (ny - 2));
+}
+
+void orc_flux_profiles(const orc_state *s, double *ttm, double *rom, double *fconm, double *fradm)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    for (int k = 1; k <= nz; k++) {
+        ttm[k - 1] = hmean_plane(s, s->tt, k);
+        rom[k - 1] = hmean_plane(s, s->ro, k);
+        fconm[k - 1] = 0.0;
+        fradm[k - 1] = 0.0;
+    }
+    const double cnt = (double)(float)((nx - 2) * (ny - 2));
+    for (int k = 2; k <= nz - 1; k++) {
+        double w = 0.0;
+        for (int j = 2; j <= ny - 1; j++)
+            for (int i = 2; i <= nx - 1; i++) {
+                const double ri = 1.0 / A(s->ro, i, j, k);
+                const double uu = A(s->ru, i, j, k) * ri, vv = A(s->rv, i, j, k) * ri, ww = A(s->rw, i, j, k) * ri;
+                const double w1 = A(s->tt, i, j, k) - ttm[k - 1];
+                w = w + (-(A(s->rw, i, j, k) * (2.5 * w1 + 0.5 * (uu * uu + vv * vv + ww * ww))));
+            }
+        fconm[k - 1] = w / cnt;
+    }
+    const double HZ = s->hz, REPR = s->repr;
+    for (int k = 3; k <= nz - 2; k++)
+        fradm[k - 1] = (ttm[k] - ttm[k - 2]) * HZ * MX(s->dzzdz, k) * MX(s->rkapa, k) / REPR;
+    {
+        const int k = 2;
+        fradm[k - 1] = (-(3.0 * ttm[k - 1]) + 4.0 * ttm[k] - ttm[k + 1]) * HZ * MX(s->dzzdz, k) * MX(s->rkapa, k) / REPR;
+    }
+    {
+        const int k = nz - 1;
+        fradm[k - 1] = (3.0 * ttm[k - 1] - 4.0 * ttm[k - 2] + ttm[k - 3]) * HZ * MX(s->dzzdz, k) * MX(s->rkapa, k) / REPR;
+    }
+}

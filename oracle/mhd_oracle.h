@@ -51,6 +51,9 @@ void orc_bcon(orc_state *s);
 void orc_communicate(orc_state *s);
 void orc_cfl(orc_state *s);
 void orc_step(orc_state *s);
+/* Horizontal means TTM, ROM and the mean convective / radiative flux
+ * profiles FCONM, FRADM (diagnostic; arrays of NZ doubles, padded K). */
+void orc_flux_profiles(const orc_state *s, double *ttm, double *rom, double *fconm, double *fradm);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,8 @@ def lib():
             getattr(l, fn).restype = None
         l.orc_check.argtypes = [C.POINTER(_OrcState)]
         l.orc_check.restype = C.c_int
+        l.orc_flux_profiles.argtypes = [C.POINTER(_OrcState)] + [_P] * 4
+        l.orc_flux_profiles.restype = None
         _lib = l
     return _lib
 
@@ -135,6 +137,15 @@ class Oracle:
     def step(self, n: int = 1):
         for _ in range(n):
             self._call("orc_step")
+
+    def flux_profiles(self) -> dict:
+        """TTM, ROM, FCONM, FRADM of the current state (padded K=1..NZ;
+        FCONM and FRADM on K=2..NZ-1, zero elsewhere)."""
+        self._sync_in()
+        nz = self.shape[2]
+        out = {k: np.zeros(nz) for k in ("ttm", "rom", "fconm", "fradm")}
+        lib().orc_flux_profiles(C.byref(self._s), *[_ptr(out[k]) for k in ("ttm", "rom", "fconm", "fradm")])
+        return out
 
     def state(self) -> dict:
         return {k: self.arrays[k].copy(order="F") for k in STATE_KEYS}

@@ -70,6 +70,7 @@ def load(path: str | None = None):
     lib.mhd_set_fixed_dt.argtypes = [C.c_void_p, C.c_double]
     lib.mhd_box_filter.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _P]
     lib.mhd_time_steps.argtypes = [C.c_void_p, C.c_int, _P, _P, _P, _P]
+    lib.mhd_flux_profiles.argtypes = [C.c_void_p, _P, _P, _P, _P]
     lib.mhd_last_error.argtypes = [C.c_void_p]
     lib.mhd_last_error.restype = C.c_char_p
     lib.mhd_destroy.argtypes = [C.c_void_p]
@@ -125,7 +126,8 @@ def load(path: str | None = None):
                "mhd_ens_get_tiling", "mhd_ens_autotune", "mhd_ens_time_steps"):
         getattr(lib, fn).restype = C.c_int
     for fn in ("mhd_create", "mhd_set_metrics", "mhd_set_background", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
-               "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps", "mhd_set_fixed_dt", "mhd_box_filter"):
+               "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps", "mhd_set_fixed_dt", "mhd_box_filter",
+               "mhd_flux_profiles"):
         getattr(lib, fn).restype = C.c_int
     _lib = lib
     return lib
@@ -267,6 +269,15 @@ class Box:
         nf = 8 if self.lmag else 5
         out = np.zeros((nf, nzc, d.npy // factor, d.npx // factor))
         self._check(self.lib.mhd_box_filter(self.h, int(factor), int(zmode), 0 if source == "state" else 1, _ptr(out)))
+        return out
+
+    def flux_profiles(self) -> dict:
+        """Horizontal means TTM, ROM and the mean convective / radiative
+        flux profiles FCONM, FRADM of the current state, one value per
+        interior plane (reference K = 2..NPZ+1)."""
+        nz = self.desc.npz
+        out = {k: np.zeros(nz) for k in ("ttm", "rom", "fconm", "fradm")}
+        self._check(self.lib.mhd_flux_profiles(self.h, *[_ptr(out[k]) for k in ("ttm", "rom", "fconm", "fradm")]))
         return out
 
     def fluxes(self) -> dict:

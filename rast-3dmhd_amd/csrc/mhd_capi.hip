@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstddef>
@@ -523,6 +524,38 @@ extern "C" int mhd_box_filter(mhd_t *h, int factor, int zmode, int source, doubl
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(h, MHD_ERR_HIP, "mhd_box_filter: %s", hipGetErrorString(e));
+    return MHD_OK;
+}
+
+extern "C" int mhd_flux_profiles(mhd_t *h, double *ttm, double *rom, double *fconm, double *fradm) {
+    int rc = ready(h, "mhd_flux_profiles");
+    if (rc) return rc;
+    const Params &p = h->p;
+    if (p.npz < 3) return fail(h, MHD_ERR_ARG, "mhd_flux_profiles: npz=%d < 3", p.npz);
+    HIPCHK(h, hipSetDevice(h->dev));
+    const int nz = p.npz;
+    std::vector<double> prof(3 * (size_t)nz);
+    double *d = nullptr;
+    HIPCHK(h, hipMalloc(&d, prof.size() * sizeof(double)));
+    hipError_t e = h->mixed ? launch_flux_profiles_mixed(p, h->hq[h->cur], h->rr, h->bg, d, h->stream)
+                            : launch_flux_profiles(p, h->q[h->cur], d, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(prof.data(), d, prof.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(h, MHD_ERR_HIP, "mhd_flux_profiles: %s", hipGetErrorString(e));
+    const double *t = prof.data();  // t[k-1]: plane k = 1..nz (reference K = k+1)
+    if (ttm) std::copy(t, t + nz, ttm);
+    if (rom) std::copy(t + nz, t + 2 * nz, rom);
+    if (fconm) std::copy(t + 2 * nz, t + 3 * nz, fconm);
+    if (fradm) {
+        // FRADM (3dmhdsub.f:1663-1677): central differences, one-sided on
+        // the two boundary planes; metric arrays are indexed by padded plane
+        const double HZ = p.hz, REPR = p.repr;
+        const double *dz = h->dzzdz.data(), *rk = h->rkapa.data();
+        for (int k = 2; k <= nz - 1; ++k) fradm[k - 1] = (t[k] - t[k - 2]) * HZ * dz[k] * rk[k] / REPR;
+        fradm[0] = (-(3.0 * t[0]) + 4.0 * t[1] - t[2]) * HZ * dz[1] * rk[1] / REPR;
+        fradm[nz - 1] = (3.0 * t[nz - 1] - 4.0 * t[nz - 2] + t[nz - 3]) * HZ * dz[nz] * rk[nz] / REPR;
+    }
     return MHD_OK;
 }
 

@@ -136,6 +136,12 @@ hipError_t launch_closure_inputs(const Params &p, const Background &bg, const fl
 hipError_t launch_closure_forcing(const Params &p, const float *y, const float *nrm_out, float *force, int *force_on,
                                   hipStream_t s);
 
+// Horizontal means and mean convective flux per interior plane (mhd_diag.hip):
+// out[3][npz] = TTM, ROM, FCONM.
+hipError_t launch_flux_profiles(const Params &p, const double *q, double *out, hipStream_t s);
+hipError_t launch_flux_profiles_mixed(const Params &p, const float *h, const unsigned *rr, const Background &bg,
+                                      double *out, hipStream_t s);
+
 // Mixed-precision form (mhd_mixed.hip).  Registers: h (FP32 stored values,
 // ping-pong), dr (FP32: residual + zeta dt F between stages 1-2 and 2-3) and
 // rr (the rounding residual of the stored values between steps as BF16, two

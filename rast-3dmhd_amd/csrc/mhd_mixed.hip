@@ -62,44 +62,11 @@ constexpr int mixed_lds_bytes() {
            (MODE == 3 ? CFL_SLOTS * MTX * TY * (int)sizeof(double) : 0);
 }
 
-// T' represented by a stored (rho', P') pair: (P' - T0 rho') / (rho0 + rho')
-// in FP32, with the operations of the stencil's ring (put_ring), so that the
-// value the residual D compensates against is bit-identical to the value
-// the right-hand side sees.
-__host__ __device__ __forceinline__ float trep(float r0, float t0, float rh, float pp) {
-    const float ri = 1.0f / (r0 + rh);
-    return (pp - t0 * rh) * ri;
-}
-
 // P' of temperature perturbation tp at density perturbation rh.  The
 // stored P' is formed from the exact (FP64) rho', not from its FP32
 // rounding: T0 times the rounding error of rho' would be comparable to P'.
 __host__ __device__ __forceinline__ double pprime(double r0, double t0, double rh, double tp) {
     return r0 * tp + t0 * rh + rh * tp;
-}
-
-// BF16 bit patterns of the resting residual R: round to nearest even
-// (NaN stays NaN), and back (exact).
-__host__ __device__ __forceinline__ unsigned short to_bf16(float x) {
-    unsigned u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__host__ __device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
-
-// R packs the BF16 residuals of two fields per 32-bit word: word
-// (f/2)*fs + o holds field f in its low (f even) or high (f odd) half.
-__device__ __forceinline__ float r_half(unsigned w, int f) {
-    return from_bf16((unsigned short)((f & 1) ? (w >> 16) : (w & 0xffffu)));
-}
-__device__ __forceinline__ float r_get(const unsigned *__restrict__ rr, long long fs, int f, long long o) {
-    return r_half(rr[(f >> 1) * fs + o], f);
-}
-__device__ __forceinline__ void r_put(unsigned *__restrict__ rr, long long fs, int f, long long o, float r) {
-    unsigned *const w = &rr[(f >> 1) * fs + o];
-    const unsigned b = to_bf16(r);
-    *w = (f & 1) ? ((*w & 0x0000ffffu) | (b << 16)) : ((*w & 0xffff0000u) | b);
 }
 
 // CFL limits of the mixed form (FP64), accumulated in monotone transformed
