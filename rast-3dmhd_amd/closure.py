@@ -495,7 +495,15 @@ if __name__ == "__main__":
     ap.add_argument("--iters", type=int, default=6000)
     ap.add_argument("--ampt", type=float, default=5e-2)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--time-net", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
+                    help="only time the network's forward pass (He-initialised weights) on an NX x NY x NZ box")
     a = ap.parse_args()
+    if a.time_net:
+        pkg = _mods()[0]
+        with pkg.Net(8, 32, 7) as net:
+            net.set_params(init_params(net.nparam, 1))
+            print(json.dumps({"net_forward_ms": net.time_forward(*a.time_net, reps=20), "box": a.time_net}))
+        sys.exit(0)
     r = run(*a.coarse, dev_factor=a.dev_factor, until_mach=a.until_mach, develop_max=a.develop_max,
             develop=a.develop, nsamples=a.samples, every=a.every, horizon=a.horizon, nsnap=a.snapshots,
             iters=a.iters, ampt=a.ampt, log=lambda *x: print(*x, file=sys.stderr, flush=True))

@@ -14,8 +14,10 @@
 //   workgroup (4 wavefronts) owns a 64 x 4 column tile and marches down a z
 //   chunk with a ring of three input planes (halo 1) in LDS; each wavefront
 //   computes one row of 64 voxels (4 M-tiles) for all output channels.  The
-//   voxel stride in LDS is an odd number of 16-byte units so the 16 lanes of
-//   a fragment read hit distinct banks.  The epilogue adds the bias and
+//   layer's weight panel is staged in LDS once per workgroup, so the inner
+//   loop reads both operands from LDS.  The voxel stride and the weight row
+//   stride in LDS are odd numbers of 16-byte units so the 16 lanes of a
+//   fragment read hit distinct banks.  The epilogue adds the bias and
 //   applies GELU (hidden layers), or multiplies by GELU'(z) (backward).
 //   The backward pass for the input reuses it with the flipped, transposed
 //   kernel.
@@ -50,6 +52,9 @@ constexpr int XS = LW + 7;                // dW: LDS row stride of X (73, odd)
 template <int CIN>
 constexpr int cstride() { return ((CIN / 8) % 2 == 0) ? CIN + 8 : CIN; }
 
+// LDS row stride (bf16) of a weight panel with rows of kp = 32 x K-blocks.
+__host__ __device__ constexpr int wstride(int kp) { return ((kp / 8) % 2 == 0) ? kp + 8 : kp; }
+
 __device__ __forceinline__ int wrapn(int x, int n) {
     int r = x % n;
     return r < 0 ? r + n : r;
@@ -62,6 +67,8 @@ __device__ __forceinline__ float gelu_grad(float x) {
 
 // EPI 0: out32 = acc + bias.  EPI 1: zout = acc + bias, out16 = gelu(zout).
 // EPI 2: out16 = acc * gelu'(zprev).  EPI 3: out32 = acc.
+// EPI 4: out16 = gelu(acc + bias) (inference: the pre-activation, which only
+// the backward pass reads, is not stored).
 template <int CIN, int NT, int EPI>
 __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const bf16 *__restrict__ wt, int kp,
                                               const float *__restrict__ bias, const float *__restrict__ zprev,
@@ -75,6 +82,8 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
     constexpr int KTOT = 27 * CIN, KB = (KTOT + 31) / 32;
     extern __shared__ uint4 lds_raw[];
     bf16 *const lds = reinterpret_cast<bf16 *>(lds_raw);
+    bf16 *const wl = lds + 3 * PLANE;  // weight panel, 16 NT rows of stride KPL
+    const int KPL = wstride(kp);
 
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 15, lg = lane >> 4;
     const int ntx = (nx + CTX - 1) / CTX, nty = (ny + CTY - 1) / CTY;
@@ -112,6 +121,11 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
             }
         }
     };
+    for (int c = t; c < 16 * NT * (kp / 8); c += 256) {
+        const int row = c / (kp / 8), part = c - row * (kp / 8);
+        *reinterpret_cast<uint4 *>(wl + row * KPL + part * 8) =
+            *reinterpret_cast<const uint4 *>(wt + (long long)row * kp + part * 8);
+    }
     for (int zz = zb - 1; zz <= zb + 1; ++zz) {
         fetch(zz);
         store(zz);
@@ -126,7 +140,7 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
+#pragma unroll 3
         for (int kb = 0; kb < KB; ++kb) {
             const int k0 = kb * 32 + lg * 8;
             bf16x8 a[4], b[NT];
@@ -142,7 +156,7 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
             }
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-                b[nt] = *reinterpret_cast<const bf16x8 *>(wt + (long long)(16 * nt + lr) * kp + k0);
+                b[nt] = *reinterpret_cast<const bf16x8 *>(wl + (16 * nt + lr) * KPL + k0);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
@@ -161,9 +175,9 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
                         const long long o = (((long long)z * ny + gy) * nx + gx) * ostride + co;
                         const float v = acc[mt][nt][r];
                         if (EPI == 0) out32[o] = v + bias[co];
-                        if (EPI == 1) {
+                        if (EPI == 1 || EPI == 4) {
                             const float zz = v + bias[co];
-                            zout[o] = zz;
+                            if (EPI == 1) zout[o] = zz;
                             out16[o] = (bf16)gelu(zz);
                         }
                         if (EPI == 2) out16[o] = (bf16)(v * gelu_grad(zprev[o]));
@@ -426,9 +440,10 @@ static int nfail(mhd_net_t *n, int code, const char *fmt, ...) {
                          __FILE__, __LINE__);                                              \
     } while (0)
 
-static int kchunk_for(int nx, int ny, int nz, int *nblocks) {
+// z chunk per workgroup so that the grid has about `target` workgroups.
+static int kchunk_for(int nx, int ny, int nz, int target, int *nblocks) {
     const int tiles = ((nx + CTX - 1) / CTX) * ((ny + CTY - 1) / CTY);
-    int nc = (1024 + tiles - 1) / tiles;
+    int nc = (target + tiles - 1) / tiles;
     if (nc > nz) nc = nz;
     if (nc < 1) nc = 1;
     const int kc = (nz + nc - 1) / nc;
@@ -439,16 +454,20 @@ static int kchunk_for(int nx, int ny, int nz, int *nblocks) {
 template <int CIN, int NT, int EPI>
 static hipError_t conv(hipStream_t s, const bf16 *x, const bf16 *w, int kp, const float *bias, const float *zprev,
                        bf16 *out16, float *out32, float *zout, int cout, int ostride, int nx, int ny, int nz) {
-    constexpr int lds = 3 * LW * LH * cstride<CIN>() * (int)sizeof(bf16);
-    static bool attr = false;
-    if (!attr) {
+    const int lds = (3 * LW * LH * cstride<CIN>() + 16 * NT * wstride(kp)) * (int)sizeof(bf16);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    static int attr = 0;
+    if (lds > attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv<CIN, NT, EPI>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
-        attr = true;
+        attr = lds;
     }
+    // as many workgroups as fit the CUs at once (LDS bound): each streams a
+    // longer z chunk, which amortises its prologue (three planes + weights)
+    const int per_cu = std::max(1, (160 * 1024) / lds);
     int nb;
-    const int kc = kchunk_for(nx, ny, nz, &nb);
+    const int kc = kchunk_for(nx, ny, nz, 256 * per_cu, &nb);
     hipLaunchKernelGGL((k_conv<CIN, NT, EPI>), dim3(nb), dim3(256), lds, s, x, w, kp, bias, zprev, out16, out32, zout,
                        cout, ostride, nx, ny, nz, kc);
     return hipGetLastError();
@@ -492,21 +511,28 @@ static hipError_t pack_weights(mhd_net_t *n) {
     return hipSuccess;
 }
 
-// forward of a BF16 input x (stride 8) into n->y (FP32, stride 8) on stream s
-static hipError_t forward_on(mhd_net_t *n, hipStream_t s, const bf16 *x, int nx, int ny, int nz) {
+// forward of a BF16 input x (stride 8) into n->y (FP32, stride 8) on stream s;
+// keep: also store the pre-activations z1, z2 the backward pass reads
+static hipError_t forward_on(mhd_net_t *n, hipStream_t s, const bf16 *x, int nx, int ny, int nz, bool keep) {
     const Layer *L = n->L;
-    hipError_t e = conv<8, 2, 1>(s, x, L[0].wf, L[0].kpf, n->p + L[0].boff, nullptr, n->a1, nullptr, n->z1, L[0].co,
-                                 L[0].cos_, nx, ny, nz);
+    hipError_t e = keep ? conv<8, 2, 1>(s, x, L[0].wf, L[0].kpf, n->p + L[0].boff, nullptr, n->a1, nullptr, n->z1,
+                                        L[0].co, L[0].cos_, nx, ny, nz)
+                        : conv<8, 2, 4>(s, x, L[0].wf, L[0].kpf, n->p + L[0].boff, nullptr, n->a1, nullptr, nullptr,
+                                        L[0].co, L[0].cos_, nx, ny, nz);
     if (e == hipSuccess)
-        e = conv<32, 2, 1>(s, n->a1, L[1].wf, L[1].kpf, n->p + L[1].boff, nullptr, n->a2, nullptr, n->z2, L[1].co,
-                           L[1].cos_, nx, ny, nz);
+        e = keep ? conv<32, 2, 1>(s, n->a1, L[1].wf, L[1].kpf, n->p + L[1].boff, nullptr, n->a2, nullptr, n->z2,
+                                  L[1].co, L[1].cos_, nx, ny, nz)
+                 : conv<32, 2, 4>(s, n->a1, L[1].wf, L[1].kpf, n->p + L[1].boff, nullptr, n->a2, nullptr, nullptr,
+                                  L[1].co, L[1].cos_, nx, ny, nz);
     if (e == hipSuccess)
         e = conv<32, 1, 0>(s, n->a2, L[2].wf, L[2].kpf, n->p + L[2].boff, nullptr, nullptr, n->y, nullptr, L[2].co, 8,
                            nx, ny, nz);
     return e;
 }
 
-static hipError_t forward(mhd_net_t *n, int nx, int ny, int nz) { return forward_on(n, n->stream, n->a0, nx, ny, nz); }
+static hipError_t forward(mhd_net_t *n, int nx, int ny, int nz, bool keep) {
+    return forward_on(n, n->stream, n->a0, nx, ny, nz, keep);
+}
 
 template <int CI, int CO>
 static hipError_t dw(mhd_net_t *n, const bf16 *x, const Layer &L, const bf16 *g, int nx, int ny, int nz) {
@@ -648,7 +674,7 @@ extern "C" int mhd_net_forward(mhd_net_t *n, int nx, int ny, int nz, const float
     if (rc) return rc;
     rc = upload_input(n, x, nvox, n->a0);
     if (rc) return rc;
-    NHIP(n, forward(n, nx, ny, nz));
+    NHIP(n, forward(n, nx, ny, nz, false));
     // y (stride 8) -> host (stride cout)
     std::vector<float> tmp((size_t)nvox * 8);
     NHIP(n, hipMemcpyAsync(tmp.data(), n->y, nvox * 8 * sizeof(float), hipMemcpyDeviceToHost, n->stream));
@@ -669,7 +695,7 @@ extern "C" int mhd_net_gradient(mhd_net_t *n, int nx, int ny, int nz, const floa
     if (rc) return rc;
     rc = upload_input(n, x, nvox, n->a0);
     if (rc) return rc;
-    NHIP(n, forward(n, nx, ny, nz));
+    NHIP(n, forward(n, nx, ny, nz, true));
     std::vector<float> g8((size_t)nvox * 8, 0.f);
     for (long long v = 0; v < nvox; ++v)
         for (int c = 0; c < n->cout; ++c) g8[(size_t)v * 8 + c] = dy[v * n->cout + c];
@@ -735,7 +761,7 @@ static int sample_step(mhd_net_t *n, int s, bool train, float lr, double *loss) 
     const Sample &S = n->samples[s];
     const long long nvox = (long long)S.nx * S.ny * S.nz;
     NHIP(n, hipMemcpyAsync(n->a0, S.x, nvox * 8 * sizeof(bf16), hipMemcpyDeviceToDevice, n->stream));
-    NHIP(n, forward(n, S.nx, S.ny, S.nz));
+    NHIP(n, forward(n, S.nx, S.ny, S.nz, train));
     const int nb = (int)std::min<long long>(1024, (nvox * 8 + 255) / 256);
     const float gscale = (float)(2.0 / ((double)nvox * n->cout));
     hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, n->stream, n->y, S.t, n->cout, nvox, gscale,
@@ -791,12 +817,12 @@ extern "C" int mhd_net_time_forward(mhd_net_t *n, int nx, int ny, int nz, int re
     NHIP(n, hipSetDevice(n->dev));
     rc = ensure_workspace(n, (long long)nx * ny * nz);
     if (rc) return rc;
-    NHIP(n, forward(n, nx, ny, nz));  // warm-up (also sets kernel attributes)
+    NHIP(n, forward(n, nx, ny, nz, false));  // warm-up (also sets kernel attributes)
     hipEvent_t a, b;
     NHIP(n, hipEventCreate(&a));
     NHIP(n, hipEventCreate(&b));
     NHIP(n, hipEventRecord(a, n->stream));
-    for (int r = 0; r < reps; ++r) NHIP(n, forward(n, nx, ny, nz));
+    for (int r = 0; r < reps; ++r) NHIP(n, forward(n, nx, ny, nz, false));
     NHIP(n, hipEventRecord(b, n->stream));
     NHIP(n, hipEventSynchronize(b));
     float t = 0;
@@ -850,7 +876,7 @@ int net_prepare(mhd_net_t *n, long long nvox, int *dev) {
 hipError_t net_forward_device(mhd_net_t *n, hipStream_t s, int nx, int ny, int nz, const __bf16 *x,
                               const float **y) {
     *y = n->y;
-    return forward_on(n, s, x, nx, ny, nz);
+    return forward_on(n, s, x, nx, ny, nz, false);
 }
 
 }  // namespace rmhd
