@@ -24,10 +24,10 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // Stored values of the mixed form (mhd_mixed.hip) as other kernels read them.
 
 // T' represented by a stored (rho', P') pair: (P' - T0 rho') / (rho0 + rho')
-// in FP32, with the operations of the stencil's ring (put_ring), so that the
-// value the residual D compensates against is bit-identical to the value
-// the right-hand side sees.
-__host__ __device__ __forceinline__ float trep(float r0, float t0, float rh, float pp) {
+// in FP32.  Every kernel that needs T' of a stored pair calls this function
+// (the stencil's ring included), so the value the residual compensates
+// against is bit-identical to the value the right-hand side sees.
+__device__ __forceinline__ float trep(float r0, float t0, float rh, float pp) {
     const float ri = 1.0f / (r0 + rh);
     return (pp - t0 * rh) * ri;
 }

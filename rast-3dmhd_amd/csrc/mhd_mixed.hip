@@ -208,7 +208,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         ring(k, 0, pt) = r.v[0] * ri;
         ring(k, 1, pt) = r.v[1] * ri;
         ring(k, 2, pt) = r.v[2] * ri;
-        ring(k, 3, pt) = (r.v[4] - bg.tt32[k] * r.v[3]) * ri;  // trep(): temperature perturbation
+        ring(k, 3, pt) = trep(bg.ro32[k], bg.tt32[k], r.v[3], r.v[4]);  // temperature perturbation
     };
 
     RawF rm_, rp_o, rp_h;
