@@ -1,0 +1,84 @@
+"""Error behaviour of the C-ABI.
+
+CPU: mhd_create validates the descriptor before any HIP call, so the
+reference's own STOP conditions and the branches that are not built come
+back as error codes with a message on a machine without a GPU.
+GPU: a state that makes the CFL limits or the Mach number non-finite is
+reported as MHD_ERR_NONFINITE (the reference carries on silently), in both
+forms and in the ensemble."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import helpers as H
+from test_gpu_parity import random_box
+
+
+def _create(meta, **over):
+    pkg = H.pkg()
+    lib = pkg.load()
+    d = pkg.make_desc(meta)
+    for k, v in over.items():
+        setattr(d, k, v)
+    h = C.c_void_p()
+    rc = lib.mhd_create(C.byref(d), C.byref(h))
+    return rc, lib.mhd_last_error(None).decode(), h
+
+
+@pytest.mark.parametrize("field,value,code,text", [
+    ("izcon", 2, "MHD_ERR_SCOPE", "Invalid IZCON"),
+    ("itcon", 3, "MHD_ERR_SCOPE", "ITCON=3"),
+    ("tp", 0.5, "MHD_ERR_SCOPE", "plume"),
+    ("precision", 7, "MHD_ERR_ARG", "precision 7"),
+    ("npz", 3, "MHD_ERR_ARG", "too small"),
+])
+def test_create_rejects_descriptors_before_touching_the_device(field, value, code, text):
+    meta, _ = random_box(16, 8, 6, True, True, seed=1)
+    rc, msg, h = _create(meta, **{field: value})
+    assert rc == getattr(H.pkg(), code), (rc, msg)
+    assert text in msg and h.value is None
+
+
+def test_null_arguments():
+    lib = H.pkg().load()
+    assert lib.mhd_create(None, None) == H.pkg().MHD_ERR_ARG
+    assert lib.mhd_step(None, None, None) == H.pkg().MHD_ERR_ARG
+    assert b"null" in lib.mhd_last_error(None)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nonfinite_state_is_reported(precision):
+    print("case non-finite state", precision, flush=True)
+    pkg = H.pkg()
+    meta, st = random_box(70, 37, 20, True, True, seed=12)
+    st["tt"][30, 20, 10] = np.nan
+    with pkg.Box(meta, precision=precision) as box:
+        box.upload(st)
+        with pytest.raises(pkg.MhdError) as e:
+            box.step()
+        assert e.value.code == pkg.MHD_ERR_NONFINITE
+        assert "non-finite" in str(e.value)
+    # a finite state steps normally again on a fresh handle
+    meta, st = random_box(70, 37, 20, True, True, seed=12)
+    with pkg.Box(meta, precision=precision) as box:
+        box.upload(st)
+        dt, um = box.step()
+        assert np.isfinite(dt) and np.isfinite(um)
+
+
+@pytest.mark.gpu
+def test_nonfinite_box_in_an_ensemble_is_reported():
+    print("case non-finite ensemble box", flush=True)
+    pkg = H.pkg()
+    m0, s0 = random_box(40, 24, 12, True, True, seed=13)
+    m1, s1 = random_box(32, 16, 10, True, True, seed=14)
+    s1["ro"][5, 5, 5] = np.nan
+    with pkg.Ensemble([m0, m1], precision="fp64") as ens:
+        ens.upload(0, s0)
+        ens.upload(1, s1)
+        with pytest.raises(pkg.MhdError) as e:
+            ens.steps(1, graph=False)
+        assert e.value.code == pkg.MHD_ERR_NONFINITE
