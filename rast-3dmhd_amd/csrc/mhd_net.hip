@@ -215,22 +215,26 @@ __global__ __launch_bounds__(256) void k_dw(const bf16 *__restrict__ x, int cis,
         const long long zy = seg / nsegx;
         const int y = (int)(zy % ny), z = (int)(zy / ny);
         __syncthreads();
-        // G of the segment, transposed: sg[co][v]
-        for (int e = t; e < CO * DSEG; e += 256) {
-            const int co = e / DSEG, v = e - co * DSEG;
+        // G of the segment, transposed: sg[co][v] (16-byte loads of 8 channels)
+        for (int e = t; e < DSEG * (CO / 8); e += 256) {
+            const int v = e / (CO / 8), c8 = e - v * (CO / 8);
             const int gx = sx0 + v;
-            float val = 0.f;
-            if (co < co_real && gx < nx) val = (float)g[(((long long)z * ny + y) * nx + gx) * cos_ + co];
-            sg[co * GS + v] = (bf16)val;
+            bf16x8 w = bf16x8{};
+            if (c8 * 8 < co_real && gx < nx)
+                w = *reinterpret_cast<const bf16x8 *>(g + (((long long)z * ny + y) * nx + gx) * cos_ + c8 * 8);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sg[(c8 * 8 + q) * GS + v] = (c8 * 8 + q < co_real) ? w[q] : (bf16)0.f;
         }
         // X rows (z+dz, y+dy), x = sx0-1 .. sx0+DSEG, transposed: sx[row][ci][x]
-        for (int e = t; e < 9 * CI * LW; e += 256) {
-            const int row = e / (CI * LW), r2 = e - row * (CI * LW);
-            const int ci = r2 / LW, col = r2 - ci * LW;
+        for (int e = t; e < 9 * LW * (CI / 8); e += 256) {
+            const int row = e / (LW * (CI / 8)), r2 = e - row * (LW * (CI / 8));
+            const int col = r2 / (CI / 8), c8 = r2 - col * (CI / 8);
             const int zz = z + row / 3 - 1, yy = wrapn(y + row % 3 - 1, ny), xx = wrapn(sx0 + col - 1, nx);
-            float val = 0.f;
-            if (ci < ci_real && zz >= 0 && zz < nz) val = (float)x[(((long long)zz * ny + yy) * nx + xx) * cis + ci];
-            sx[(row * CI + ci) * XS + col] = (bf16)val;
+            bf16x8 w = bf16x8{};
+            if (c8 * 8 < ci_real && zz >= 0 && zz < nz)
+                w = *reinterpret_cast<const bf16x8 *>(x + (((long long)zz * ny + yy) * nx + xx) * cis + c8 * 8);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sx[(row * CI + c8 * 8 + q) * XS + col] = (c8 * 8 + q < ci_real) ? w[q] : (bf16)0.f;
         }
         __syncthreads();
 #pragma unroll
@@ -275,43 +279,74 @@ __global__ __launch_bounds__(256) void k_dw(const bf16 *__restrict__ x, int cis,
     }
 }
 
-// dW[co][tap][ci] (params order) = sum over blocks, in block order.
-__global__ void k_dw_reduce(const float *__restrict__ partial, int nblk, int co_pad, int ci_pad, int co_real,
-                            int ci_real, float *__restrict__ dw) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= co_real * 27 * ci_real) return;
-    const int co = e / (27 * ci_real), r = e - co * 27 * ci_real, tap = r / ci_real, ci = r - tap * ci_real;
+// dW[co][tap][ci] (params order) = sum over the block partials: a workgroup
+// takes 32 consecutive outputs, its 8 thread groups sum the blocks c, c+8,
+// c+16, ... and the 8 group sums are added in order (deterministic).
+constexpr int DWR_OUT = 32, DWR_GRP = 8;
+__global__ __launch_bounds__(DWR_OUT *DWR_GRP) void k_dw_reduce(const float *__restrict__ partial, int nblk, int co_pad,
+                                                               int ci_pad, int co_real, int ci_real,
+                                                               float *__restrict__ dw) {
+    __shared__ float red[DWR_GRP][DWR_OUT];
+    const int j = threadIdx.x % DWR_OUT, c = threadIdx.x / DWR_OUT;
+    const int e = blockIdx.x * DWR_OUT + j;
+    const int ne = co_real * 27 * ci_real;
     float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += partial[(((long long)b * 27 + tap) * co_pad + co) * ci_pad + ci];
-    dw[e] = s;
-}
-
-// db partials: block b sums G over its voxel range, per channel.
-__global__ __launch_bounds__(256) void k_db(const bf16 *__restrict__ g, int cos_, int co_real, long long nvox,
-                                            float *__restrict__ partial) {
-    __shared__ float red[256];
-    const long long per = (nvox + gridDim.x - 1) / gridDim.x;
-    const long long v0 = blockIdx.x * per, v1 = min(nvox, v0 + per);
-    for (int co = 0; co < co_real; ++co) {
-        float s = 0.f;
-        for (long long v = v0 + threadIdx.x; v < v1; v += 256) s += (float)g[v * cos_ + co];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) partial[blockIdx.x * co_real + co] = red[0];
-        __syncthreads();
+    if (e < ne) {
+        const int co = e / (27 * ci_real), r = e - co * 27 * ci_real, tap = r / ci_real, ci = r - tap * ci_real;
+        for (int b = c; b < nblk; b += DWR_GRP) s += partial[(((long long)b * 27 + tap) * co_pad + co) * ci_pad + ci];
+    }
+    red[c][j] = s;
+    __syncthreads();
+    if (c == 0 && e < ne) {
+        float t = red[0][j];
+        for (int q = 1; q < DWR_GRP; ++q) t += red[q][j];
+        dw[e] = t;
     }
 }
 
-__global__ void k_db_reduce(const float *__restrict__ partial, int nblk, int co_real, float *__restrict__ db) {
-    const int co = threadIdx.x;
-    if (co >= co_real) return;
+// db partials: block b sums G over its voxel range.  A thread loads 8
+// channels (16 bytes) at a time; the per-channel sums of the threads are
+// then added in thread order (deterministic).
+__global__ __launch_bounds__(256) void k_db(const bf16 *__restrict__ g, int cos_, int co_real, long long nvox,
+                                            float *__restrict__ partial) {
+    __shared__ float red[256 * 8];
+    const int nch = cos_ / 8, vpb = 256 / nch;  // 16-byte chunks per voxel, voxels per pass
+    const int t = threadIdx.x, ch = t % nch, vt = t / nch;
+    const long long per = (nvox + gridDim.x - 1) / gridDim.x;
+    const long long v0 = blockIdx.x * per, v1 = min(nvox, v0 + per);
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (long long v = v0 + vt; v < v1; v += vpb) {
+        const bf16x8 w = *reinterpret_cast<const bf16x8 *>(g + v * cos_ + ch * 8);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] += (float)w[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) red[t * 8 + q] = acc[q];
+    __syncthreads();
+    if (t < co_real) {
+        const int cc = t / 8, q = t % 8;
+        float s = 0.f;
+        for (int u = 0; u < vpb; ++u) s += red[(u * nch + cc) * 8 + q];
+        partial[blockIdx.x * co_real + t] = s;
+    }
+}
+
+// db = sum over the block partials: 8 thread groups sum the blocks c, c+8,
+// ..., then the group sums are added in order (deterministic).
+__global__ __launch_bounds__(DWR_OUT *DWR_GRP) void k_db_reduce(const float *__restrict__ partial, int nblk,
+                                                               int co_real, float *__restrict__ db) {
+    __shared__ float red[DWR_GRP][DWR_OUT];
+    const int co = threadIdx.x % DWR_OUT, c = threadIdx.x / DWR_OUT;
     float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += partial[b * co_real + co];
-    db[co] = s;
+    if (co < co_real)
+        for (int b = c; b < nblk; b += DWR_GRP) s += partial[b * co_real + co];
+    red[c][co] = s;
+    __syncthreads();
+    if (c == 0 && co < co_real) {
+        float t = red[0][co];
+        for (int q = 1; q < DWR_GRP; ++q) t += red[q][co];
+        db[co] = t;
+    }
 }
 
 // BF16 weight panels of one layer from the FP32 master W[co][tap][ci]:
@@ -543,8 +578,8 @@ static hipError_t dw(mhd_net_t *n, const bf16 *x, const Layer &L, const bf16 *g,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int ne = L.co * 27 * L.ci;
-    hipLaunchKernelGGL(k_dw_reduce, dim3((ne + 255) / 256), dim3(256), 0, n->stream, n->partial, nblk, CO, CI, L.co,
-                       L.ci, n->grad + L.woff);
+    hipLaunchKernelGGL(k_dw_reduce, dim3((ne + DWR_OUT - 1) / DWR_OUT), dim3(DWR_OUT * DWR_GRP), 0, n->stream,
+                       n->partial, nblk, CO, CI, L.co, L.ci, n->grad + L.woff);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     const long long nvox = (long long)nx * ny * nz;
@@ -552,7 +587,8 @@ static hipError_t dw(mhd_net_t *n, const bf16 *x, const Layer &L, const bf16 *g,
     hipLaunchKernelGGL(k_db, dim3(nb), dim3(256), 0, n->stream, g, L.cos_, L.co, nvox, n->partial);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_db_reduce, dim3(1), dim3(64), 0, n->stream, n->partial, nb, L.co, n->grad + L.boff);
+    hipLaunchKernelGGL(k_db_reduce, dim3(1), dim3(DWR_OUT * DWR_GRP), 0, n->stream, n->partial, nb, L.co,
+                       n->grad + L.boff);
     return hipGetLastError();
 }
 
