@@ -20,6 +20,27 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
     return x * q + (x < r ? x : r) + local;
 }
 
+// Entry i of a read-only per-plane table (grid metrics, background
+// profiles) at a workgroup-uniform index: read through the constant
+// address space, so the compiler issues a scalar load.  A vector load here
+// would share the in-order vector-memory counter with the plane prefetch
+// and the wait for it would also wait for every load issued before it.
+// The tables are written only by host copies between kernels.
+template <class T>
+__device__ __forceinline__ T uload(const T *p, int i) {
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
+
+// x, held back until y has been computed: an empty asm that reads y and
+// may change x, so that nothing derived from x is scheduled before y.  Used
+// to keep the first use of a loaded value (and so the wait for the load,
+// which also waits for every load issued before it) late in a plane.
+template <class X, class Y>
+__device__ __forceinline__ X after(X x, Y y) {
+    asm volatile("" : "+v"(x) : "v"(y));
+    return x;
+}
 
 // Stored values of the mixed form (mhd_mixed.hip) as other kernels read them.
 

@@ -33,6 +33,7 @@
 // and writes h and R: 112 + 128 + 112 = 352 B/zone/step (FP64: 640).
 
 #include "mhd_device.h"
+#include <type_traits>
 
 namespace rmhd {
 
@@ -204,11 +205,11 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         for (int f = 0; f < NF; ++f) cur(k, f, pt) = r.v[f];
     };
     auto put_ring = [&](int k, int pt, const RawF &r) {
-        const float ri = 1.0f / (bg.ro32[k] + r.v[3]);
+        const float ri = 1.0f / (uload(bg.ro32, k) + r.v[3]);
         ring(k, 0, pt) = r.v[0] * ri;
         ring(k, 1, pt) = r.v[1] * ri;
         ring(k, 2, pt) = r.v[2] * ri;
-        ring(k, 3, pt) = trep(bg.ro32[k], bg.tt32[k], r.v[3], r.v[4]);  // temperature perturbation
+        ring(k, 3, pt) = trep(uload(bg.ro32, k), uload(bg.tt32, k), r.v[3], r.v[4]);  // temperature perturbation
     };
 
     RawF rm_, rp_o, rp_h;
@@ -253,33 +254,36 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
     }
     const bool frc = bg.force != nullptr && *bg.force_on != 0;
 
-#pragma unroll 1
-    for (int k = kbeg; k <= kend; ++k) {
+    // One plane of the march.  BND instantiates the boundary-plane form
+    // (one-sided z stencils, extra planes read directly); the interior form
+    // has none of that code, so its loads are waited for only where used.
+    auto plane = [&](const int k, auto bnd) __attribute__((always_inline)) {
+        constexpr bool BND = decltype(bnd)::value;
         const bool pre = k < kend;
         const long long o = (long long)k * pl + ooff;
         if (pre) {
             load_rawf<MAG>(hin, fs, (long long)(k + 2) * pl + ooff, rp_o);
             if (has_h) load_rawf<MAG>(hin, fs, (long long)(k + 2) * pl + hoff, rp_h);
         }
+        // D (stages 2, 3) or the packed R words (stage 1) of this point, read
+        // once; R is unpacked where it is used, so that no wait for these
+        // loads comes before the plane's arithmetic
         float dk_[8];
+        unsigned rk_[4];
         if (valid) {
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {  // read once
+            for (int f = 0; f < NF; ++f) {
                 if (MODE != 1) dk_[f] = __builtin_nontemporal_load(&dr[f * fs + o]);
-                else if (!(f & 1)) {
-                    const unsigned w = __builtin_nontemporal_load(&rr[(f >> 1) * fs + o]);
-                    dk_[f] = r_half(w, f);
-                    if (f + 1 < NF) dk_[f + 1] = r_half(w, f + 1);
-                }
+                else if (!(f & 1)) rk_[f >> 1] = __builtin_nontemporal_load(&rr[(f >> 1) * fs + o]);
             }
         }
         __syncthreads();
         if (valid) {
-            const bool top = (k == 1), bot = (k == npz);
-            const rhs_t dz = (rhs_t)m.dzzdz[k], d2z = (rhs_t)m.d2zzdz2[k];
-            const rhs_t rk = (rhs_t)m.rkapa[k], dkap = (rhs_t)m.dkapa[k];
+            const bool top = BND && k == 1, bot = BND && k == npz;
+            const rhs_t dz = (rhs_t)uload(m.dzzdz, k), d2z = (rhs_t)uload(m.d2zzdz2, k);
+            const rhs_t rk = (rhs_t)uload(m.rkapa, k), dkap = (rhs_t)uload(m.dkapa, k);
             const rhs_t hzdz = HZ * dz, hzd2z = HZ * d2z, h2zdz2 = H2Z * dz * dz;
-            const rhs_t r0 = bg.ro32[k], t0 = bg.tt32[k];
+            const rhs_t r0 = uload(bg.ro32, k), t0 = uload(bg.tt32, k);
 #define CUR(f, d) cur(k, f, opt + (d))
 #define NXT(f) cur(k + 1, f, opt)
 #define RNG(kp, c, d) ring(kp, c, opt + (d))
@@ -312,7 +316,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 const int kb = top ? k + 3 : k - 3;
                 const long long oa = (long long)ka * pl + ooff, ob = (long long)kb * pl + ooff;
                 rwk2 = hin[2 * fs + oa];
-                wk2 = rwk2 * (rhs_t(1) / ((rhs_t)bg.ro32[ka] + (rhs_t)hin[3 * fs + oa]));
+                wk2 = rwk2 * (rhs_t(1) / ((rhs_t)uload(bg.ro32, ka) + (rhs_t)hin[3 * fs + oa]));
                 if (MAG) {
                     bxk2 = hin[5 * fs + oa];
                     byk2 = hin[6 * fs + oa];
@@ -340,14 +344,14 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             // temperature advection, the background gradient included
             const rhs_t txp = RNG(k, 3, 1), txm = RNG(k, 3, -1), typ = RNG(k, 3, SW), tym = RNG(k, 3, -SW);
             const rhs_t dtz = (th_p - th_m) * hzdz;
-            rhs_t ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + bg.dt0dz32[k]));
+            rhs_t ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + uload(bg.dt0dz32, k)));
             // conduction of T': (1/rho) c [kappa lap T' + kappa' dT'/dz]; plus the
             // change of the background's conduction with 1/rho: -rho'/(rho rho0) S0
             {
                 const rhs_t lap = (txp - txm) * hxd2x + (txp - rhs_t(2) * th0 + txm) * h2xdx2
                                   + (typ - tym) * hyd2y + (typ - rhs_t(2) * th0 + tym) * h2ydy2
                                   + (th_p - th_m) * hzd2z + (th_p - rhs_t(2) * th0 + th_m) * h2zdz2;
-                ft += rinv * (rhs_t)p.t_cond * (rk * lap + dkap * dtz) - rh0 * rinv * bg.rinv0_32[k] * bg.s0_32[k];
+                ft += rinv * (rhs_t)p.t_cond * (rk * lap + dkap * dtz) - rh0 * rinv * uload(bg.rinv0_32, k) * uload(bg.s0_32, k);
             }
             // viscous Laplacians
             fu += ORE * (C43 * ((uxp - uxm) * hxd2x + (uxp - rhs_t(2) * u0 + uxm) * h2xdx2)
@@ -434,8 +438,8 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             // (the temperature slot's value is T'(rho', P')); store the new
             // FP32 values and carry the residual (+ zeta dt F) in D.
             double Fd[8] = {fu, fv, fw, fr, ft, w1, w2, w3};
-            Fd[2] += bg.fwbg[k];
-            Fd[4] += bg.ftbg[k];
+            Fd[2] += uload(bg.fwbg, k);
+            Fd[4] += uload(bg.ftbg, k);
             if (frc) {
                 const float *g = bg.force + o;
                 Fd[0] += (double)g[0];
@@ -448,20 +452,22 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                     Fd[7] += (double)g[6 * fs];
                 }
             }
-            const double r0d = bg.ro[k], t0d = bg.tt[k];
+            const double r0d = uload(bg.ro, k), t0d = uload(bg.tt, k);
             double xq[8];
             float hn[8];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const double v = f == 4 ? (double)th0 : (double)h0[f];
-                xq[f] = (v + (double)dk_[f]) + coefg * Fd[f];
+                // (read only once this field's right-hand side exists)
+                const float dkf = MODE == 1 ? r_half(after(rk_[f >> 1], Fd[f]), f) : after(dk_[f], Fd[f]);
+                xq[f] = (v + (double)dkf) + coefg * Fd[f];
                 if (f != 4) hn[f] = (float)xq[f];
             }
             hn[4] = (float)pprime(r0d, t0d, xq[3], xq[4]);
             unsigned rw = 0;
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                const double r = xq[f] - (f == 4 ? (double)trep(bg.ro32[k], bg.tt32[k], hn[3], hn[4]) : (double)hn[f]);
+                const double r = xq[f] - (f == 4 ? (double)trep(uload(bg.ro32, k), uload(bg.tt32, k), hn[3], hn[4]) : (double)hn[f]);
                 __builtin_nontemporal_store(hn[f], &hout[f * fs + o]);
                 if (MODE < 3) {
                     __builtin_nontemporal_store((float)(r + coefz * Fd[f]), &dr[f * fs + o]);
@@ -475,12 +481,12 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 double full[8];
 #pragma unroll
                 for (int f = 0; f < NF; ++f) full[f] = xq[f];
-                full[3] += bg.ro[k];
-                full[4] += bg.tt[k];
-                const DdPair d = ddpair(c_ddx, c_ddy, m.ddz[k]);
+                full[3] += uload(bg.ro, k);
+                full[4] += uload(bg.tt, k);
+                const DdPair d = ddpair(c_ddx, c_ddy, uload(m.ddz, k));
                 CflFast c{s_cfl[0 * T::NT + t], s_cfl[1 * T::NT + t], s_cfl[2 * T::NT + t], s_cfl[3 * T::NT + t],
                           INFINITY};
-                cflf_point<MAG>(p, full, d.dd2, d.idd2, 1.0 / m.rkapa[k], c);
+                cflf_point<MAG>(p, full, d.dd2, d.idd2, 1.0 / uload(m.rkapa, k), c);
                 s_cfl[0 * T::NT + t] = c.mach;
                 s_cfl[1 * T::NT + t] = c.q1;
                 s_cfl[2 * T::NT + t] = c.s2;
@@ -498,6 +504,11 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 put_ring(k + 2, hpt, rp_h);
             }
         }
+    };
+#pragma unroll 1
+    for (int k = kbeg; k <= kend; ++k) {
+        if (k == 1 || k == npz) plane(k, std::true_type{});
+        else plane(k, std::false_type{});
     }
     if (MODE == 3) {
         // min(dd^2) depends on the grid only: its minimum over all points
@@ -509,7 +520,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
 }
 
 template <int TY, bool MAG, int MODE>
-__global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metrics m, const Background bg,
+__global__ __launch_bounds__(MTX *TY, 3) void k_stage_m(const Params p, const Metrics m, const Background bg,
                                                      const float *__restrict__ hin, float *__restrict__ hout,
                                                      float *__restrict__ dr, unsigned *__restrict__ rr,
                                                      DevScalars *__restrict__ sc, int kchunk) {
@@ -527,7 +538,7 @@ __global__ __launch_bounds__(MTX *TY) void k_stage_m(const Params p, const Metri
 
 // Ensemble: persistent grid over the work items of all boxes (static stride).
 template <int TY, bool MAG, int MODE>
-__global__ __launch_bounds__(MTX *TY) void k_stage_m_batch(const BoxDev *__restrict__ boxes,
+__global__ __launch_bounds__(MTX *TY, 3) void k_stage_m_batch(const BoxDev *__restrict__ boxes,
                                                            const StageItem *__restrict__ items, int nitems, int a) {
     extern __shared__ float ldsf[];
     const int slot = xcd_remap(blockIdx.x, gridDim.x);
