@@ -42,6 +42,14 @@ __device__ __forceinline__ X after(X x, Y y) {
     return x;
 }
 
+// x is needed here: an empty asm that reads x, so the wait for the load of
+// x happens at this point (once, before a loop) rather than at its first use
+// inside the loop, where it would be repeated on every iteration.
+template <class X>
+__device__ __forceinline__ void settle(X x) {
+    asm volatile("" ::"v"(x));
+}
+
 // Stored values of the mixed form (mhd_mixed.hip) as other kernels read them.
 
 // T' represented by a stored (rho', P') pair: (P' - T0 rho') / (rho0 + rho')

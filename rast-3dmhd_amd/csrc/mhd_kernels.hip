@@ -32,6 +32,7 @@
 // are bit-identical to the reference's FP64 build.
 
 #include "mhd_device.h"
+#include <type_traits>
 
 namespace rmhd {
 
@@ -225,25 +226,32 @@ __device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, co
     // CFL accumulators of the new state (stage 3 only)
     double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
     const double c_ddx = MODE == 3 ? m.ddx[gix] : 0.0, c_ddy = MODE == 3 ? m.ddy[gjy] : 0.0;
+    // the per-column metrics arrive before the march, not inside its first plane
+    settle(dx), settle(d2x), settle(dy), settle(d2y), settle(c_ddx), settle(c_ddy);
 
-#pragma unroll 1
-    for (int k = kbeg; k <= kend; ++k) {
+    // One plane of the march.  BND instantiates the boundary-plane form
+    // (one-sided z stencils, extra planes read directly); the interior form
+    // has none of that code, so its loads are waited for only where used.
+    auto plane = [&](const int k, auto bnd) __attribute__((always_inline)) {
+        constexpr bool BND = decltype(bnd)::value;
         const bool pre = k < kend;
         const long long o = (long long)k * pl + ooff;
         if (pre) {
             load_raw<MAG>(qin, fs, (long long)(k + 2) * pl + ooff, rp_o);
             if (has_h) load_raw<MAG>(qin, fs, (long long)(k + 2) * pl + hoff, rp_h);
         }
+        // Z of this point, read once (every lane: the wrapped column is in
+        // range also for lanes outside the domain)
         double zk[8];
-        if (MODE >= 2 && valid) {
+        if (MODE >= 2) {
 #pragma unroll
-            for (int f = 0; f < NF; ++f) zk[f] = __builtin_nontemporal_load(&zr[f * fs + o]);  // read once
+            for (int f = 0; f < NF; ++f) zk[f] = __builtin_nontemporal_load(&zr[f * fs + o]);
         }
         __syncthreads();
         if (valid) {
-            const bool top = (k == 1), bot = (k == npz);
-            const double dz = m.dzzdz[k], d2z = m.d2zzdz2[k];
-            const double rk = m.rkapa[k], dk = m.dkapa[k];
+            const bool top = BND && k == 1, bot = BND && k == npz;
+            const double dz = uload(m.dzzdz, k), d2z = uload(m.d2zzdz2, k);
+            const double rk = uload(m.rkapa, k), dk = uload(m.dkapa, k);
             const double tmpz = HZ * dz;
             const double tmpz1 = HZ * d2z;
             const double tmpz2 = H2Z * dz * dz;
@@ -493,17 +501,25 @@ __device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, co
                 double qn[8];
 #pragma unroll
                 for (int f = 0; f < NF; ++f) {
-                    const double zin = (MODE == 1) ? q0[f] : zk[f];
+                    // (Z is read only once this field's right-hand side exists)
+                    const double zin = (MODE == 1) ? q0[f] : after(zk[f], F[f]);
                     qn[f] = zin + coefg * F[f];
                     __builtin_nontemporal_store(qn[f], &qout[f * fs + o]);
                     if (MODE < 3) __builtin_nontemporal_store(qn[f] + coefz * F[f], &zr[f * fs + o]);
                 }
                 if (MODE == 3 && !top && !bot)
-                    cfl_point<MAG>(p, qn, c_ddx, c_ddy, m.ddz[k], rk, c_mach, c_w1, c_w2, c_w3, c_w4);
+                    cfl_point<MAG>(p, qn, c_ddx, c_ddy, uload(m.ddz, k), rk, c_mach, c_w1, c_w2, c_w3, c_w4);
             }
             // own raw values of plane k become the k-1 values of the next plane
 #pragma unroll
             for (int f = 0; f < NF; ++f) rm_.v[f] = q0[f];
+            if (BND) settle(rwk2), settle(wk2), settle(bxk2), settle(byk2), settle(bxk3), settle(byk3);
+        }
+        // every load of this plane has arrived on every path out of it, so
+        // none is still counted as outstanding in the next plane
+        if (MODE >= 2) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f) settle(zk[f]);
         }
         __syncthreads();
         // plane k+2 replaces plane k in LDS and enters the ring
@@ -515,13 +531,20 @@ __device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, co
                 put_ring(k + 2, hpt, rp_h);
             }
         }
-    }
+    };
+    int k0 = kbeg, k1 = kend;
+    if (k0 == 1) plane(k0++, std::true_type{});
+    const bool last_bnd = k1 == npz && k1 >= k0;
+    if (last_bnd) --k1;
+#pragma unroll 1
+    for (int k = k0; k <= k1; ++k) plane(k, std::false_type{});
+    if (last_bnd) plane(npz, std::true_type{});
     if (MODE == 3) cfl_reduce(c_mach, c_w1, c_w2, c_w3, c_w4, p.lmag, sc);
 }
 
 // Single box: workgroup -> (tile, z chunk), XCD-aware.
 template <int TY, bool MAG, int MODE>
-__global__ __launch_bounds__(TX *TY) void k_stage(const Params p, const Metrics m,
+__global__ __launch_bounds__(TX *TY, 2) void k_stage(const Params p, const Metrics m,
                                                   const double *__restrict__ qin,
                                                   double *__restrict__ qout,
                                                   double *__restrict__ zr,
@@ -544,7 +567,7 @@ __global__ __launch_bounds__(TX *TY) void k_stage(const Params p, const Metrics 
 // static stride (deterministic; every workgroup leaves after its last item).
 // Register a of every box is the input, 1-a the output.
 template <int TY, bool MAG, int MODE>
-__global__ __launch_bounds__(TX *TY) void k_stage_batch(const BoxDev *__restrict__ boxes,
+__global__ __launch_bounds__(TX *TY, 2) void k_stage_batch(const BoxDev *__restrict__ boxes,
                                                         const StageItem *__restrict__ items, int nitems, int a) {
     extern __shared__ double lds[];
     const int slot = xcd_remap(blockIdx.x, gridDim.x);
