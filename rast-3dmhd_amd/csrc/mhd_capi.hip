@@ -43,7 +43,7 @@ struct mhd_ctx {
     double *stage = nullptr;            // one padded field for host transfers
     double *met = nullptr;              // 11 metric arrays
     Metrics m{};
-    std::vector<double> dzzdz, d2zzdz2, rkapa, dkapa;  // host copies for the background
+    std::vector<double> dzzdz, d2zzdz2, ddz, rkapa, dkapa;  // host copies for the background
     DevScalars *sc = nullptr;
     // closure-corrected step (mixed form)
     mhd_net_t *net = nullptr;
@@ -126,8 +126,8 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
         if (e == hipSuccess) e = hipMalloc(&h->hq[1], fbytes);
         if (e == hipSuccess) e = hipMalloc(&h->dr, fbytes);
         if (e == hipSuccess) e = hipMalloc(&h->rr, resid_bytes(nf, p.fsize));
-        if (e == hipSuccess) e = hipMalloc(&h->bgd, 4 * nzp * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&h->bgf, 5 * nzp * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&h->bgd, BG_DBL * nzp * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&h->bgf, BG_FLT * nzp * sizeof(float));
     }
     if (e == hipSuccess) e = hipMalloc(&h->stage, pbytes);
     if (e == hipSuccess) e = hipMalloc(&h->met, sizeof(double) * (3 * (d->npx + 2) + 3 * (d->npy + 2) + 5 * (d->npz + 2)));
@@ -173,6 +173,7 @@ extern "C" int mhd_set_metrics(mhd_t *h, const double *dxxdx, const double *d2xx
     const size_t nz = (size_t)h->p.npz + 2;
     h->dzzdz.assign(dzzdz, dzzdz + nz);
     h->d2zzdz2.assign(d2zzdz2, d2zzdz2 + nz);
+    h->ddz.assign(ddz, ddz + nz);
     h->rkapa.assign(rkapa, rkapa + nz);
     h->dkapa.assign(dkapa, dkapa + nz);
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -187,8 +188,8 @@ extern "C" int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0
     if (!ro0 || !tt0) return fail(h, MHD_ERR_ARG, "mhd_set_background: null array");
     std::vector<double> d;
     std::vector<float> f;
-    const int bad = background_tables(h->p, h->dzzdz.data(), h->d2zzdz2.data(), h->rkapa.data(), h->dkapa.data(),
-                                      ro0, tt0, d, f);
+    const int bad = background_tables(h->p, h->dzzdz.data(), h->d2zzdz2.data(), h->ddz.data(), h->rkapa.data(),
+                                      h->dkapa.data(), ro0, tt0, d, f);
     if (bad >= 0) return fail(h, MHD_ERR_ARG, "mhd_set_background: rho0[%d] <= 0", bad);
     HIPCHK(h, hipSetDevice(h->dev));
     HIPCHK(h, hipMemcpyAsync(h->bgd, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));

@@ -33,7 +33,7 @@ struct EnsBox {
     mhd_desc desc{};
     Params p{};
     size_t reg_elems = 0;            // elements of one register (nf * fsize)
-    std::vector<double> dzzdz, d2zzdz2, rkapa, dkapa;
+    std::vector<double> dzzdz, d2zzdz2, ddz, rkapa, dkapa;
     bool have_metrics = false, have_bg = false, have_state = false;
     bool cfl_valid = false;
 };
@@ -189,8 +189,8 @@ extern "C" int mhd_ens_create(const mhd_desc *descs, int nbox, mhd_ens_t **out) 
     if (r == hipSuccess) r = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
     if (r == hipSuccess) r = hipMalloc(&e->pool, pool);
     if (r == hipSuccess) r = hipMalloc(&e->met, met * sizeof(double));
-    if (r == hipSuccess && e->mixed) r = hipMalloc(&e->bgd, 4 * nzsum * sizeof(double));
-    if (r == hipSuccess && e->mixed) r = hipMalloc(&e->bgf, 5 * nzsum * sizeof(float));
+    if (r == hipSuccess && e->mixed) r = hipMalloc(&e->bgd, BG_DBL * nzsum * sizeof(double));
+    if (r == hipSuccess && e->mixed) r = hipMalloc(&e->bgf, BG_FLT * nzsum * sizeof(float));
     if (r == hipSuccess) r = hipMalloc(&e->sc, nbox * sizeof(DevScalars));
     if (r == hipSuccess) r = hipMalloc(&e->stage, e->stage_bytes);
     if (r == hipSuccess) r = hipMalloc(&e->dbox, nbox * sizeof(BoxDev));
@@ -215,8 +215,8 @@ extern "C" int mhd_ens_create(const mhd_desc *descs, int nbox, mhd_ens_t **out) 
             D.dr = reinterpret_cast<float *>(base + 2 * rb);
             D.rr = reinterpret_cast<unsigned *>(base + 3 * rb);
             const size_t o = bg_off[b];
-            // 4 FP64 and 5 FP32 tables of npz+2 entries, laid out per box
-            D.bg = background_view(B.p.npz, e->bgd + 4 * o, e->bgf + 5 * o);
+            // BG_DBL FP64 and BG_FLT FP32 tables of npz+2 entries, laid out per box
+            D.bg = background_view(B.p.npz, e->bgd + BG_DBL * o, e->bgf + BG_FLT * o);
         } else {
             D.q[0] = reinterpret_cast<double *>(base);
             D.q[1] = reinterpret_cast<double *>(base + rb);
@@ -266,6 +266,7 @@ extern "C" int mhd_ens_set_metrics(mhd_ens_t *e, int b, const double *dxxdx, con
     const size_t nz = (size_t)B.p.npz + 2;
     B.dzzdz.assign(dzzdz, dzzdz + nz);
     B.d2zzdz2.assign(d2zzdz2, d2zzdz2 + nz);
+    B.ddz.assign(ddz, ddz + nz);
     B.rkapa.assign(rkapa, rkapa + nz);
     B.dkapa.assign(dkapa, dkapa + nz);
     EHIP(e, hipStreamSynchronize(e->stream));
@@ -282,7 +283,7 @@ extern "C" int mhd_ens_set_background(mhd_ens_t *e, int b, const double *ro0, co
     if (!ro0 || !tt0) return efail(e, MHD_ERR_ARG, "mhd_ens_set_background: null array");
     std::vector<double> d;
     std::vector<float> f;
-    const int bad = background_tables(B.p, B.dzzdz.data(), B.d2zzdz2.data(), B.rkapa.data(), B.dkapa.data(), ro0, tt0,
+    const int bad = background_tables(B.p, B.dzzdz.data(), B.d2zzdz2.data(), B.ddz.data(), B.rkapa.data(), B.dkapa.data(), ro0, tt0,
                                       d, f);
     if (bad >= 0) return efail(e, MHD_ERR_ARG, "mhd_ens_set_background: box %d: rho0[%d] <= 0", b, bad);
     EHIP(e, hipSetDevice(e->dev));

@@ -124,14 +124,18 @@ inline Metrics metric_table(const Params &p, const double *base, size_t len[11])
 // rounding errors of rho0 and T0 would leave a systematic error in it.
 // fwbg, ftbg are the background's own discrete residuals of the momentum z
 // and temperature equations (conduction S0 / rho0), formed in FP64 with the
-// same stencils as the step.  Returns the first plane with rho0 <= 0, or -1.
-inline int background_tables(const Params &p, const double *dzzdz, const double *d2zzdz2, const double *rkapa,
-                             const double *dkapa, const double *ro0, const double *tt0, std::vector<double> &dbl,
-                             std::vector<float> &flt) {
+// same stencils as the step.  iddz2 = 1/ddz^2 and irk = 1/rkapa are per-plane
+// factors of the CFL limits, divided here once instead of per point (IEEE
+// division on host and device alike).  Returns the first plane with
+// rho0 <= 0, or -1.
+constexpr int BG_DBL = 6, BG_FLT = 5;  // FP64 and FP32 tables of npz+2 entries
+inline int background_tables(const Params &p, const double *dzzdz, const double *d2zzdz2, const double *ddz,
+                             const double *rkapa, const double *dkapa, const double *ro0, const double *tt0,
+                             std::vector<double> &dbl, std::vector<float> &flt) {
     const int nz = p.npz + 2;
-    dbl.assign(4 * (size_t)nz, 0.0);
-    flt.assign(5 * (size_t)nz, 0.0f);
-    double *ro = dbl.data(), *tt = ro + nz, *fw = tt + nz, *ft = fw + nz;
+    dbl.assign(BG_DBL * (size_t)nz, 0.0);
+    flt.assign(BG_FLT * (size_t)nz, 0.0f);
+    double *ro = dbl.data(), *tt = ro + nz, *fw = tt + nz, *ft = fw + nz, *iz = ft + nz, *irk = iz + nz;
     float *ro32 = flt.data(), *tt32 = ro32 + nz, *dtz = tt32 + nz, *ri = dtz + nz, *s0 = ri + nz;
     for (int k = 0; k < nz; ++k) {
         if (!(ro0[k] > 0.0) || !((float)ro0[k] > 0.0f)) return k;
@@ -150,6 +154,8 @@ inline int background_tables(const Params &p, const double *dzzdz, const double 
         s0[k] = (float)sz;
         ft[k] = sz / ro[k];
         fw[k] = p.grav * ro[k] - (ro[k + 1] * tt[k + 1] - ro[k - 1] * tt[k - 1]) * p.hz * dz;
+        iz[k] = 1.0 / (ddz[k] * ddz[k]);
+        irk[k] = 1.0 / rkapa[k];
     }
     return -1;
 }
@@ -162,6 +168,8 @@ inline Background background_view(int npz, const double *bgd, const float *bgf) 
     bg.tt = bgd + nz;
     bg.fwbg = bgd + 2 * nz;
     bg.ftbg = bgd + 3 * nz;
+    bg.iddz2 = bgd + 4 * nz;
+    bg.irk = bgd + 5 * nz;
     bg.ro32 = bgf;
     bg.tt32 = bgf + nz;
     bg.dt0dz32 = bgf + 2 * nz;

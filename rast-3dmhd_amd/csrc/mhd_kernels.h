@@ -62,6 +62,7 @@ struct DevScalars {
 // coefficients for the perturbation stencil.
 struct Background {
     const double *ro, *tt, *fwbg, *ftbg;
+    const double *iddz2, *irk;  // 1/ddz^2 and 1/rkapa per plane (grid only; the CFL of stage 3)
     const float *ro32, *tt32, *dt0dz32, *rinv0_32, *s0_32;
     // optional subgrid forcing of the closure-corrected step: FP32 fields
     // [RU,RV,RW,TT,BX,BY,BZ][plane][j][i] added to the right-hand side of

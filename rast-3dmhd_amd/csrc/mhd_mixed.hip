@@ -244,7 +244,15 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
     const double dt = sc->dt;
     const double coefg = (MODE == 1 ? p.gam1 : MODE == 2 ? p.gam2 : p.gam3) * dt;
     const double coefz = (MODE == 1 ? p.zeta1 : p.zeta2) * dt;
-    const double c_ddx = MODE == 3 ? m.ddx[gix] : 0.0, c_ddy = MODE == 3 ? m.ddy[gjy] : 0.0;
+    // stage 3: this column's part of min(dd^2) and its inverse; per plane
+    // the CFL takes dd^2 = min(c_d2, ddz^2) and 1/dd^2 = max(c_id2, 1/ddz^2),
+    // which is 1/min(...) exactly (correctly rounded division is monotone)
+    double c_d2 = 0.0, c_id2 = 0.0;
+    if (MODE == 3) {
+        const double ddx = m.ddx[gix], ddy = m.ddy[gjy];
+        c_d2 = fmin(ddx * ddx, ddy * ddy);
+        c_id2 = 1.0 / c_d2;
+    }
     if (MODE == 3) {
         const CflFast c0 = cflf_init();
         s_cfl[0 * T::NT + t] = c0.mach;
@@ -483,10 +491,11 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 for (int f = 0; f < NF; ++f) full[f] = xq[f];
                 full[3] += uload(bg.ro, k);
                 full[4] += uload(bg.tt, k);
-                const DdPair d = ddpair(c_ddx, c_ddy, uload(m.ddz, k));
+                const double ddz = uload(m.ddz, k);
+                const double dd2 = fmin(c_d2, ddz * ddz), idd2 = fmax(c_id2, uload(bg.iddz2, k));
                 CflFast c{s_cfl[0 * T::NT + t], s_cfl[1 * T::NT + t], s_cfl[2 * T::NT + t], s_cfl[3 * T::NT + t],
                           INFINITY};
-                cflf_point<MAG>(p, full, d.dd2, d.idd2, 1.0 / uload(m.rkapa, k), c);
+                cflf_point<MAG>(p, full, dd2, idd2, uload(bg.irk, k), c);
                 s_cfl[0 * T::NT + t] = c.mach;
                 s_cfl[1 * T::NT + t] = c.q1;
                 s_cfl[2 * T::NT + t] = c.s2;
