@@ -38,6 +38,7 @@ struct mhd_ctx {
     unsigned *rr = nullptr;             // mixed: BF16 residual pairs (between steps)
     double *bgd = nullptr;              // mixed: FP64 background arrays
     float *bgf = nullptr;               // mixed: FP32 background arrays
+    PlaneM *bgp = nullptr;              // mixed: per-plane records of the stage kernel
     Background bg{};
     double *f = nullptr;                // right-hand side (mhd_fluxes only)
     double *stage = nullptr;            // one padded field for host transfers
@@ -84,7 +85,7 @@ static void free_all(mhd_t *h) {
     if (!h) return;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *bufs[17] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->rr, h->bgd, h->bgf,
+    void *bufs[18] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->rr, h->bgd, h->bgf, h->bgp,
                       h->f, h->stage, h->met, h->sc, h->force, h->force_on, h->cx, h->nrm};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -128,6 +129,7 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
         if (e == hipSuccess) e = hipMalloc(&h->rr, resid_bytes(nf, p.fsize));
         if (e == hipSuccess) e = hipMalloc(&h->bgd, BG_DBL * nzp * sizeof(double));
         if (e == hipSuccess) e = hipMalloc(&h->bgf, BG_FLT * nzp * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&h->bgp, nzp * sizeof(PlaneM));
     }
     if (e == hipSuccess) e = hipMalloc(&h->stage, pbytes);
     if (e == hipSuccess) e = hipMalloc(&h->met, sizeof(double) * (3 * (d->npx + 2) + 3 * (d->npy + 2) + 5 * (d->npz + 2)));
@@ -188,14 +190,17 @@ extern "C" int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0
     if (!ro0 || !tt0) return fail(h, MHD_ERR_ARG, "mhd_set_background: null array");
     std::vector<double> d;
     std::vector<float> f;
+    std::vector<PlaneM> pl;
     const int bad = background_tables(h->p, h->dzzdz.data(), h->d2zzdz2.data(), h->ddz.data(), h->rkapa.data(),
-                                      h->dkapa.data(), ro0, tt0, d, f);
+                                      h->dkapa.data(), ro0, tt0, d, f, pl);
     if (bad >= 0) return fail(h, MHD_ERR_ARG, "mhd_set_background: rho0[%d] <= 0", bad);
     HIPCHK(h, hipSetDevice(h->dev));
     HIPCHK(h, hipMemcpyAsync(h->bgd, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->bgf, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->bgp, pl.data(), pl.size() * sizeof(PlaneM), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->bg = background_view(h->p.npz, h->bgd, h->bgf);
+    h->bg.planes = h->bgp;
     if (h->closure) {
         h->bg.force = h->force;
         h->bg.force_on = h->force_on;

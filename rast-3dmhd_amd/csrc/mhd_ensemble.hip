@@ -54,6 +54,7 @@ struct mhd_ens {
     double *met = nullptr;           // metric blocks
     double *bgd = nullptr;           // mixed: FP64 background blocks
     float *bgf = nullptr;            // mixed: FP32 background blocks
+    PlaneM *bgp = nullptr;           // mixed: per-plane records of the stage kernel, per box
     DevScalars *sc = nullptr;
     double *stage = nullptr;         // one padded field of the largest box
     size_t stage_bytes = 0;
@@ -101,7 +102,7 @@ static void free_ens(mhd_ens_t *e) {
     (void)hipSetDevice(e->dev);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
-    void *bufs[9] = {e->dbox, e->pool, e->met, e->bgd, e->bgf, e->sc, e->stage, e->ditems, e->dcols};
+    void *bufs[10] = {e->dbox, e->pool, e->met, e->bgd, e->bgf, e->bgp, e->sc, e->stage, e->ditems, e->dcols};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -191,6 +192,7 @@ extern "C" int mhd_ens_create(const mhd_desc *descs, int nbox, mhd_ens_t **out) 
     if (r == hipSuccess) r = hipMalloc(&e->met, met * sizeof(double));
     if (r == hipSuccess && e->mixed) r = hipMalloc(&e->bgd, BG_DBL * nzsum * sizeof(double));
     if (r == hipSuccess && e->mixed) r = hipMalloc(&e->bgf, BG_FLT * nzsum * sizeof(float));
+    if (r == hipSuccess && e->mixed) r = hipMalloc(&e->bgp, nzsum * sizeof(PlaneM));
     if (r == hipSuccess) r = hipMalloc(&e->sc, nbox * sizeof(DevScalars));
     if (r == hipSuccess) r = hipMalloc(&e->stage, e->stage_bytes);
     if (r == hipSuccess) r = hipMalloc(&e->dbox, nbox * sizeof(BoxDev));
@@ -217,6 +219,7 @@ extern "C" int mhd_ens_create(const mhd_desc *descs, int nbox, mhd_ens_t **out) 
             const size_t o = bg_off[b];
             // BG_DBL FP64 and BG_FLT FP32 tables of npz+2 entries, laid out per box
             D.bg = background_view(B.p.npz, e->bgd + BG_DBL * o, e->bgf + BG_FLT * o);
+            D.bg.planes = e->bgp + o;
         } else {
             D.q[0] = reinterpret_cast<double *>(base);
             D.q[1] = reinterpret_cast<double *>(base + rb);
@@ -283,13 +286,16 @@ extern "C" int mhd_ens_set_background(mhd_ens_t *e, int b, const double *ro0, co
     if (!ro0 || !tt0) return efail(e, MHD_ERR_ARG, "mhd_ens_set_background: null array");
     std::vector<double> d;
     std::vector<float> f;
-    const int bad = background_tables(B.p, B.dzzdz.data(), B.d2zzdz2.data(), B.ddz.data(), B.rkapa.data(), B.dkapa.data(), ro0, tt0,
-                                      d, f);
+    std::vector<PlaneM> pl;
+    const int bad = background_tables(B.p, B.dzzdz.data(), B.d2zzdz2.data(), B.ddz.data(), B.rkapa.data(),
+                                      B.dkapa.data(), ro0, tt0, d, f, pl);
     if (bad >= 0) return efail(e, MHD_ERR_ARG, "mhd_ens_set_background: box %d: rho0[%d] <= 0", b, bad);
     EHIP(e, hipSetDevice(e->dev));
     EHIP(e, hipMemcpyAsync(const_cast<double *>(e->hbox[b].bg.ro), d.data(), d.size() * sizeof(double),
                            hipMemcpyHostToDevice, e->stream));
     EHIP(e, hipMemcpyAsync(const_cast<float *>(e->hbox[b].bg.ro32), f.data(), f.size() * sizeof(float),
+                           hipMemcpyHostToDevice, e->stream));
+    EHIP(e, hipMemcpyAsync(const_cast<PlaneM *>(e->hbox[b].bg.planes), pl.data(), pl.size() * sizeof(PlaneM),
                            hipMemcpyHostToDevice, e->stream));
     EHIP(e, hipStreamSynchronize(e->stream));
     B.have_bg = true;

@@ -131,7 +131,7 @@ inline Metrics metric_table(const Params &p, const double *base, size_t len[11])
 constexpr int BG_DBL = 6, BG_FLT = 5;  // FP64 and FP32 tables of npz+2 entries
 inline int background_tables(const Params &p, const double *dzzdz, const double *d2zzdz2, const double *ddz,
                              const double *rkapa, const double *dkapa, const double *ro0, const double *tt0,
-                             std::vector<double> &dbl, std::vector<float> &flt) {
+                             std::vector<double> &dbl, std::vector<float> &flt, std::vector<PlaneM> &pl) {
     const int nz = p.npz + 2;
     dbl.assign(BG_DBL * (size_t)nz, 0.0);
     flt.assign(BG_FLT * (size_t)nz, 0.0f);
@@ -156,6 +156,26 @@ inline int background_tables(const Params &p, const double *dzzdz, const double 
         fw[k] = p.grav * ro[k] - (ro[k + 1] * tt[k + 1] - ro[k - 1] * tt[k - 1]) * p.hz * dz;
         iz[k] = 1.0 / (ddz[k] * ddz[k]);
         irk[k] = 1.0 / rkapa[k];
+    }
+    pl.assign((size_t)nz, PlaneM{});
+    for (int k = 0; k < nz; ++k) {
+        PlaneM &q = pl[k];
+        q.dzzdz = dzzdz[k];
+        q.d2zzdz2 = d2zzdz2[k];
+        q.rkapa = rkapa[k];
+        q.dkapa = dkapa[k];
+        q.ddz = ddz[k];
+        q.ro = ro[k];
+        q.tt = tt[k];
+        q.fwbg = fw[k];
+        q.ftbg = ft[k];
+        q.iddz2 = iz[k];
+        q.irk = irk[k];
+        q.ro32 = ro32[k];
+        q.tt32 = tt32[k];
+        q.dt0dz32 = dtz[k];
+        q.rinv0_32 = ri[k];
+        q.s0_32 = s0[k];
     }
     return -1;
 }

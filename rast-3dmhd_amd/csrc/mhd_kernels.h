@@ -60,9 +60,21 @@ struct DevScalars {
 // FP64 rho0, T0 and the background's own discrete residuals of the momentum
 // z equation and the temperature equation; FP32 copies and per-plane
 // coefficients for the perturbation stencil.
+// Everything the mixed stage kernel reads per plane (metrics and
+// background), as one 128-byte record: the z march holds one table pointer
+// instead of one per array, and reads the fields it needs through scalar
+// loads where it uses them.
+struct PlaneM {
+    double dzzdz, d2zzdz2, rkapa, dkapa, ddz, ro, tt, fwbg, ftbg, iddz2, irk;
+    float ro32, tt32, dt0dz32, rinv0_32, s0_32;
+    float pad_[5];
+};
+static_assert(sizeof(PlaneM) == 128, "PlaneM is one 128-byte record");
+
 struct Background {
     const double *ro, *tt, *fwbg, *ftbg;
     const double *iddz2, *irk;  // 1/ddz^2 and 1/rkapa per plane (grid only; the CFL of stage 3)
+    const PlaneM *planes;       // the same per plane with the metrics, for the stage kernel
     const float *ro32, *tt32, *dt0dz32, *rinv0_32, *s0_32;
     // optional subgrid forcing of the closure-corrected step: FP32 fields
     // [RU,RV,RW,TT,BX,BY,BZ][plane][j][i] added to the right-hand side of

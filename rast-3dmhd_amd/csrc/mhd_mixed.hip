@@ -162,7 +162,15 @@ __device__ __forceinline__ void load_rawf(const float *__restrict__ q, long long
 
 // One tile column chunk of mixed stage s = MODE (1..3); see stage_tile of
 // the FP64 form for the tile organisation.
-template <int TY, bool MAG, int MODE>
+// Per-plane values: PLANES reads them from the 128-byte records of
+// Background::planes (one table pointer), else from the separate metric and
+// background arrays.  The records lower the scalar-register pressure of the
+// march; the ensemble's stage-3 kernel, whose spills then land in its
+// plane loop, is the one instantiation faster with the arrays.
+#define PLM(F, K) (PLANES ? uload(&bg.planes[K].F, 0) : uload(m.F, K))
+#define PLB(F, K) (PLANES ? uload(&bg.planes[K].F, 0) : uload(bg.F, K))
+
+template <int TY, bool MAG, int MODE, bool PLANES>
 __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, const Background &bg,
                                              const float *__restrict__ hin, float *__restrict__ hout,
                                              float *__restrict__ dr, unsigned *__restrict__ rr,
@@ -205,11 +213,11 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         for (int f = 0; f < NF; ++f) cur(k, f, pt) = r.v[f];
     };
     auto put_ring = [&](int k, int pt, const RawF &r) {
-        const float ri = 1.0f / (uload(bg.ro32, k) + r.v[3]);
+        const float ri = 1.0f / (PLB(ro32, k) + r.v[3]);
         ring(k, 0, pt) = r.v[0] * ri;
         ring(k, 1, pt) = r.v[1] * ri;
         ring(k, 2, pt) = r.v[2] * ri;
-        ring(k, 3, pt) = trep(uload(bg.ro32, k), uload(bg.tt32, k), r.v[3], r.v[4]);  // temperature perturbation
+        ring(k, 3, pt) = trep(PLB(ro32, k), PLB(tt32, k), r.v[3], r.v[4]);  // temperature perturbation
     };
 
     RawF rm_, rp_o, rp_h;
@@ -288,10 +296,10 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         __syncthreads();
         if (valid) {
             const bool top = BND && k == 1, bot = BND && k == npz;
-            const rhs_t dz = (rhs_t)uload(m.dzzdz, k), d2z = (rhs_t)uload(m.d2zzdz2, k);
-            const rhs_t rk = (rhs_t)uload(m.rkapa, k), dkap = (rhs_t)uload(m.dkapa, k);
+            const rhs_t dz = (rhs_t)PLM(dzzdz, k), d2z = (rhs_t)PLM(d2zzdz2, k);
+            const rhs_t rk = (rhs_t)PLM(rkapa, k), dkap = (rhs_t)PLM(dkapa, k);
             const rhs_t hzdz = HZ * dz, hzd2z = HZ * d2z, h2zdz2 = H2Z * dz * dz;
-            const rhs_t r0 = uload(bg.ro32, k), t0 = uload(bg.tt32, k);
+            const rhs_t r0 = PLB(ro32, k), t0 = PLB(tt32, k);
 #define CUR(f, d) cur(k, f, opt + (d))
 #define NXT(f) cur(k + 1, f, opt)
 #define RNG(kp, c, d) ring(kp, c, opt + (d))
@@ -324,7 +332,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 const int kb = top ? k + 3 : k - 3;
                 const long long oa = (long long)ka * pl + ooff, ob = (long long)kb * pl + ooff;
                 rwk2 = hin[2 * fs + oa];
-                wk2 = rwk2 * (rhs_t(1) / ((rhs_t)uload(bg.ro32, ka) + (rhs_t)hin[3 * fs + oa]));
+                wk2 = rwk2 * (rhs_t(1) / ((rhs_t)PLB(ro32, ka) + (rhs_t)hin[3 * fs + oa]));
                 if (MAG) {
                     bxk2 = hin[5 * fs + oa];
                     byk2 = hin[6 * fs + oa];
@@ -352,14 +360,14 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             // temperature advection, the background gradient included
             const rhs_t txp = RNG(k, 3, 1), txm = RNG(k, 3, -1), typ = RNG(k, 3, SW), tym = RNG(k, 3, -SW);
             const rhs_t dtz = (th_p - th_m) * hzdz;
-            rhs_t ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + uload(bg.dt0dz32, k)));
+            rhs_t ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + PLB(dt0dz32, k)));
             // conduction of T': (1/rho) c [kappa lap T' + kappa' dT'/dz]; plus the
             // change of the background's conduction with 1/rho: -rho'/(rho rho0) S0
             {
                 const rhs_t lap = (txp - txm) * hxd2x + (txp - rhs_t(2) * th0 + txm) * h2xdx2
                                   + (typ - tym) * hyd2y + (typ - rhs_t(2) * th0 + tym) * h2ydy2
                                   + (th_p - th_m) * hzd2z + (th_p - rhs_t(2) * th0 + th_m) * h2zdz2;
-                ft += rinv * (rhs_t)p.t_cond * (rk * lap + dkap * dtz) - rh0 * rinv * uload(bg.rinv0_32, k) * uload(bg.s0_32, k);
+                ft += rinv * (rhs_t)p.t_cond * (rk * lap + dkap * dtz) - rh0 * rinv * PLB(rinv0_32, k) * PLB(s0_32, k);
             }
             // viscous Laplacians
             fu += ORE * (C43 * ((uxp - uxm) * hxd2x + (uxp - rhs_t(2) * u0 + uxm) * h2xdx2)
@@ -446,8 +454,8 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             // (the temperature slot's value is T'(rho', P')); store the new
             // FP32 values and carry the residual (+ zeta dt F) in D.
             double Fd[8] = {fu, fv, fw, fr, ft, w1, w2, w3};
-            Fd[2] += uload(bg.fwbg, k);
-            Fd[4] += uload(bg.ftbg, k);
+            Fd[2] += PLB(fwbg, k);
+            Fd[4] += PLB(ftbg, k);
             if (frc) {
                 const float *g = bg.force + o;
                 Fd[0] += (double)g[0];
@@ -460,7 +468,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                     Fd[7] += (double)g[6 * fs];
                 }
             }
-            const double r0d = uload(bg.ro, k), t0d = uload(bg.tt, k);
+            const double r0d = PLB(ro, k), t0d = PLB(tt, k);
             double xq[8];
             float hn[8];
 #pragma unroll
@@ -475,7 +483,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             unsigned rw = 0;
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                const double r = xq[f] - (f == 4 ? (double)trep(uload(bg.ro32, k), uload(bg.tt32, k), hn[3], hn[4]) : (double)hn[f]);
+                const double r = xq[f] - (f == 4 ? (double)trep(PLB(ro32, k), PLB(tt32, k), hn[3], hn[4]) : (double)hn[f]);
                 __builtin_nontemporal_store(hn[f], &hout[f * fs + o]);
                 if (MODE < 3) {
                     __builtin_nontemporal_store((float)(r + coefz * Fd[f]), &dr[f * fs + o]);
@@ -489,13 +497,13 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 double full[8];
 #pragma unroll
                 for (int f = 0; f < NF; ++f) full[f] = xq[f];
-                full[3] += uload(bg.ro, k);
-                full[4] += uload(bg.tt, k);
-                const double ddz = uload(m.ddz, k);
-                const double dd2 = fmin(c_d2, ddz * ddz), idd2 = fmax(c_id2, uload(bg.iddz2, k));
+                full[3] += PLB(ro, k);
+                full[4] += PLB(tt, k);
+                const double ddz = PLM(ddz, k);
+                const double dd2 = fmin(c_d2, ddz * ddz), idd2 = fmax(c_id2, PLB(iddz2, k));
                 CflFast c{s_cfl[0 * T::NT + t], s_cfl[1 * T::NT + t], s_cfl[2 * T::NT + t], s_cfl[3 * T::NT + t],
                           INFINITY};
-                cflf_point<MAG>(p, full, dd2, idd2, uload(bg.irk, k), c);
+                cflf_point<MAG>(p, full, dd2, idd2, PLB(irk, k), c);
                 s_cfl[0 * T::NT + t] = c.mach;
                 s_cfl[1 * T::NT + t] = c.q1;
                 s_cfl[2 * T::NT + t] = c.s2;
@@ -542,7 +550,7 @@ __global__ __launch_bounds__(MTX *TY, 3) void k_stage_m(const Params p, const Me
     const int kbeg = 1 + tz * kchunk;
     const int kend = min(p.npz, kbeg + kchunk - 1);
     if (kbeg > p.npz) return;
-    stage_tile_m<TY, MAG, MODE>(p, m, bg, hin, hout, dr, rr, sc, txb * MTX, tyb * TY, kbeg, kend, ldsf);
+    stage_tile_m<TY, MAG, MODE, true>(p, m, bg, hin, hout, dr, rr, sc, txb * MTX, tyb * TY, kbeg, kend, ldsf);
 }
 
 // Ensemble: persistent grid over the work items of all boxes (static stride).
@@ -555,8 +563,8 @@ __global__ __launch_bounds__(MTX *TY, 3) void k_stage_m_batch(const BoxDev *__re
         const StageItem it = items[w];
         const BoxDev &B = boxes[it.box];
         __syncthreads();  // the previous item's last LDS reads are done
-        stage_tile_m<TY, MAG, MODE>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.rr, B.sc, it.i0, it.j0, it.kbeg,
-                                    it.kend, ldsf);
+        stage_tile_m<TY, MAG, MODE, MODE != 3>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.rr, B.sc, it.i0, it.j0,
+                                               it.kbeg, it.kend, ldsf);
     }
 }
 
