@@ -176,6 +176,19 @@ inline int background_tables(const Params &p, const double *dzzdz, const double 
         q.dt0dz32 = dtz[k];
         q.rinv0_32 = ri[k];
         q.s0_32 = s0[k];
+        q.c43 = (float)p.c43;
+        q.ocv = (float)p.ocv;
+        q.ore = (float)p.ore;
+        q.orm = (float)p.orm;
+        q.obeta = (float)p.obeta;
+        q.grav = (float)p.grav;
+        q.t_cond = (float)p.t_cond;
+        q.t_visc = (float)p.t_visc;
+        q.t_visc43 = (float)p.t_visc43;
+        q.t_cross = (float)p.t_cross;
+        q.t_ohm = (float)p.t_ohm;
+        q.omx = (float)p.omx;
+        q.omz = (float)p.omz;
     }
     return -1;
 }

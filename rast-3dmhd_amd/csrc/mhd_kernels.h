@@ -61,15 +61,18 @@ struct DevScalars {
 // z equation and the temperature equation; FP32 copies and per-plane
 // coefficients for the perturbation stencil.
 // Everything the mixed stage kernel reads per plane (metrics and
-// background), as one 128-byte record: the z march holds one table pointer
+// background), as one 160-byte record: the z march holds one table pointer
 // instead of one per array, and reads the fields it needs through scalar
-// loads where it uses them.
+// loads where it uses them.  The FP32 coefficients of the right-hand side
+// (the same in every record) are read there too: loads at plane-dependent
+// addresses stay in the plane instead of holding scalar registers for the
+// whole march.
 struct PlaneM {
     double dzzdz, d2zzdz2, rkapa, dkapa, ddz, ro, tt, fwbg, ftbg, iddz2, irk;
     float ro32, tt32, dt0dz32, rinv0_32, s0_32;
-    float pad_[5];
+    float c43, ocv, ore, orm, obeta, grav, t_cond, t_visc, t_visc43, t_cross, t_ohm, omx, omz;
 };
-static_assert(sizeof(PlaneM) == 128, "PlaneM is one 128-byte record");
+static_assert(sizeof(PlaneM) == 160, "PlaneM is one 160-byte record");
 
 struct Background {
     const double *ro, *tt, *fwbg, *ftbg;

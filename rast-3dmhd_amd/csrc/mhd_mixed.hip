@@ -162,15 +162,13 @@ __device__ __forceinline__ void load_rawf(const float *__restrict__ q, long long
 
 // One tile column chunk of mixed stage s = MODE (1..3); see stage_tile of
 // the FP64 form for the tile organisation.
-// Per-plane values: PLANES reads them from the 128-byte records of
-// Background::planes (one table pointer), else from the separate metric and
-// background arrays.  The records lower the scalar-register pressure of the
-// march; the ensemble's stage-3 kernel, whose spills then land in its
-// plane loop, is the one instantiation faster with the arrays.
-#define PLM(F, K) (PLANES ? uload(&bg.planes[K].F, 0) : uload(m.F, K))
-#define PLB(F, K) (PLANES ? uload(&bg.planes[K].F, 0) : uload(bg.F, K))
+// Per-plane values (z metrics, background, coefficients) from the
+// records of Background::planes: one table pointer for the march.
+#define PLM(F, K) uload(&bg.planes[K].F, 0)
+#define PLB(F, K) uload(&bg.planes[K].F, 0)
+#define PLP(F, K) uload(&bg.planes[K].F, 0)
 
-template <int TY, bool MAG, int MODE, bool PLANES>
+template <int TY, bool MAG, int MODE>
 __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, const Background &bg,
                                              const float *__restrict__ hin, float *__restrict__ hout,
                                              float *__restrict__ dr, unsigned *__restrict__ rr,
@@ -241,8 +239,6 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
 
     const rhs_t HX = (rhs_t)p.hx, H2X = (rhs_t)p.h2x, HY = (rhs_t)p.hy, H2Y = (rhs_t)p.h2y;
     const rhs_t HZ = (rhs_t)p.hz, H2Z = (rhs_t)p.h2z;
-    const rhs_t C43 = (rhs_t)p.c43, OCV = (rhs_t)p.ocv, ORE = (rhs_t)p.ore, ORM = (rhs_t)p.orm;
-    const rhs_t OBETA = (rhs_t)p.obeta, GRAV = (rhs_t)p.grav;
     const int gix = wrapi(gi, npx) + 1, gjy = wrapi(gj, npy) + 1;
     const rhs_t dx = (rhs_t)m.dxxdx[gix], d2x = (rhs_t)m.d2xxdx2[gix];
     const rhs_t dy = (rhs_t)m.dyydy[gjy], d2y = (rhs_t)m.d2yydy2[gjy];
@@ -296,6 +292,8 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         __syncthreads();
         if (valid) {
             const bool top = BND && k == 1, bot = BND && k == npz;
+            const rhs_t C43 = PLP(c43, k), OCV = PLP(ocv, k), ORE = PLP(ore, k), ORM = PLP(orm, k);
+            const rhs_t OBETA = PLP(obeta, k), GRAV = PLP(grav, k);
             const rhs_t dz = (rhs_t)PLM(dzzdz, k), d2z = (rhs_t)PLM(d2zzdz2, k);
             const rhs_t rk = (rhs_t)PLM(rkapa, k), dkap = (rhs_t)PLM(dkapa, k);
             const rhs_t hzdz = HZ * dz, hzd2z = HZ * d2z, h2zdz2 = H2Z * dz * dz;
@@ -367,7 +365,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 const rhs_t lap = (txp - txm) * hxd2x + (txp - rhs_t(2) * th0 + txm) * h2xdx2
                                   + (typ - tym) * hyd2y + (typ - rhs_t(2) * th0 + tym) * h2ydy2
                                   + (th_p - th_m) * hzd2z + (th_p - rhs_t(2) * th0 + th_m) * h2zdz2;
-                ft += rinv * (rhs_t)p.t_cond * (rk * lap + dkap * dtz) - rh0 * rinv * PLB(rinv0_32, k) * PLB(s0_32, k);
+                ft += rinv * PLP(t_cond, k) * (rk * lap + dkap * dtz) - rh0 * rinv * PLB(rinv0_32, k) * PLB(s0_32, k);
             }
             // viscous Laplacians
             fu += ORE * (C43 * ((uxp - uxm) * hxd2x + (uxp - rhs_t(2) * u0 + uxm) * h2xdx2)
@@ -380,7 +378,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                          + (wyp - wym) * hyd2y + (wyp - rhs_t(2) * w0 + wym) * h2ydy2
                          + C43 * ((wp - wm) * hzd2z + (wp - rhs_t(2) * w0 + wm) * h2zdz2));
             // heating and grad div u cross terms
-            const rhs_t tv = (rhs_t)p.t_visc, tv43 = (rhs_t)p.t_visc43, tcr = (rhs_t)p.t_cross;
+            const rhs_t tv = PLP(t_visc, k), tv43 = PLP(t_visc43, k), tcr = PLP(t_cross, k);
             const rhs_t dxu = (uxp - uxm) * hxdx, dxv = (vxp - vxm) * hxdx, dxw = (wxp - wxm) * hxdx;
             const rhs_t dyu = (uyp - uym) * hydy, dyv = (vyp - vym) * hydy, dyw = (wyp - wym) * hydy;
             const rhs_t dzu = (up - um) * hzdz, dzv = (vp - vm) * hzdz, dzw = (wp - wm) * hzdz;
@@ -396,7 +394,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                          + ((RNG(k + 1, 1, SW) - RNG(k - 1, 1, SW)) - (RNG(k + 1, 1, -SW) - RNG(k - 1, 1, -SW))) * hzdz * hydy);
             // Coriolis
             if (p.lrot) {
-                const rhs_t omx = (rhs_t)p.omx, omz = (rhs_t)p.omz;
+                const rhs_t omx = PLP(omx, k), omz = PLP(omz, k);
                 fu += omz * q0[1];
                 fv += -omz * q0[0] + omx * q0[2];
                 fw -= omx * q0[1];
@@ -426,7 +424,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 fw += OBETA * (bx0 * dxbz - bx0 * dzbx - by0 * (dzby - dybz));
                 // Ohmic heating |J|^2 / rho
                 ft += ((dybx - dxby) * (dybx - dxby) + dxbz * dxbz + dzbx * dzbx - rhs_t(2) * dzbx * dxbz
-                       + (dzby - dybz) * (dzby - dybz)) * rinv * (rhs_t)p.t_ohm;
+                       + (dzby - dybz) * (dzby - dybz)) * rinv * PLP(t_ohm, k);
                 // resistive diffusion, one-sided in z on the boundary planes for BX, BY
                 rhs_t d2zbx, d2zby;
                 if (top) {
@@ -550,7 +548,7 @@ __global__ __launch_bounds__(MTX *TY, 3) void k_stage_m(const Params p, const Me
     const int kbeg = 1 + tz * kchunk;
     const int kend = min(p.npz, kbeg + kchunk - 1);
     if (kbeg > p.npz) return;
-    stage_tile_m<TY, MAG, MODE, true>(p, m, bg, hin, hout, dr, rr, sc, txb * MTX, tyb * TY, kbeg, kend, ldsf);
+    stage_tile_m<TY, MAG, MODE>(p, m, bg, hin, hout, dr, rr, sc, txb * MTX, tyb * TY, kbeg, kend, ldsf);
 }
 
 // Ensemble: persistent grid over the work items of all boxes (static stride).
@@ -563,8 +561,8 @@ __global__ __launch_bounds__(MTX *TY, 3) void k_stage_m_batch(const BoxDev *__re
         const StageItem it = items[w];
         const BoxDev &B = boxes[it.box];
         __syncthreads();  // the previous item's last LDS reads are done
-        stage_tile_m<TY, MAG, MODE, MODE != 3>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.rr, B.sc, it.i0, it.j0,
-                                               it.kbeg, it.kend, ldsf);
+        stage_tile_m<TY, MAG, MODE>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.rr, B.sc, it.i0, it.j0, it.kbeg,
+                                    it.kend, ldsf);
     }
 }
 
