@@ -54,13 +54,10 @@ struct MTile {
 };
 
 // LDS: raw planes k, k+1 (2 x NF fields) and a three-plane ring of u, v, w,
-// T'; in stage 3 also each thread's running CFL values (4 doubles), which
-// would otherwise stay live in registers across the whole z march.
-constexpr int CFL_SLOTS = 4;
+// T'.
 template <int TY, bool MAG, int MODE>
 constexpr int mixed_lds_bytes() {
-    return (2 * (MAG ? 8 : 5) + 12) * MTile<TY>::SP * (int)sizeof(float) +
-           (MODE == 3 ? CFL_SLOTS * MTX * TY * (int)sizeof(double) : 0);
+    return (2 * (MAG ? 8 : 5) + 12) * MTile<TY>::SP * (int)sizeof(float);
 }
 
 // P' of temperature perturbation tp at density perturbation rh.  The
@@ -179,8 +176,6 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
     constexpr int NF = MAG ? 8 : 5;
     float *const s_cur = ldsf;
     float *const s_ring = ldsf + 2 * NF * SP;
-    // stage 3: running CFL maxima/minima of this thread, [slot][thread]
-    double *const s_cfl = reinterpret_cast<double *>(ldsf + (2 * NF + 12) * SP);
 
     const int t = threadIdx.x;
     const int npx = p.npx, npy = p.npy, npz = p.npz;
@@ -257,13 +252,8 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         c_d2 = fmin(ddx * ddx, ddy * ddy);
         c_id2 = 1.0 / c_d2;
     }
-    if (MODE == 3) {
-        const CflFast c0 = cflf_init();
-        s_cfl[0 * T::NT + t] = c0.mach;
-        s_cfl[1 * T::NT + t] = c0.q1;
-        s_cfl[2 * T::NT + t] = c0.s2;
-        s_cfl[3 * T::NT + t] = c0.s3;
-    }
+    // stage 3: this thread's running CFL maxima/minima over its column
+    CflFast cfl_acc = cflf_init();
     const bool frc = bg.force != nullptr && *bg.force_on != 0;
 
     // One plane of the march.  BND instantiates the boundary-plane form
@@ -499,13 +489,10 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 full[4] += PLB(tt, k);
                 const double ddz = PLM(ddz, k);
                 const double dd2 = fmin(c_d2, ddz * ddz), idd2 = fmax(c_id2, PLB(iddz2, k));
-                CflFast c{s_cfl[0 * T::NT + t], s_cfl[1 * T::NT + t], s_cfl[2 * T::NT + t], s_cfl[3 * T::NT + t],
-                          INFINITY};
+                // (min(dd^2) is not accumulated per point: see the reduction)
+                CflFast c{cfl_acc.mach, cfl_acc.q1, cfl_acc.s2, cfl_acc.s3, INFINITY};
                 cflf_point<MAG>(p, full, dd2, idd2, PLB(irk, k), c);
-                s_cfl[0 * T::NT + t] = c.mach;
-                s_cfl[1 * T::NT + t] = c.q1;
-                s_cfl[2 * T::NT + t] = c.s2;
-                s_cfl[3 * T::NT + t] = c.s3;
+                cfl_acc = c;
             }
 #pragma unroll
             for (int f = 0; f < NF; ++f) rm_.v[f] = h0[f];
@@ -528,9 +515,8 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
     if (MODE == 3) {
         // min(dd^2) depends on the grid only: its minimum over all points
         // comes with the metrics
-        const CflFast c{s_cfl[0 * T::NT + t], s_cfl[1 * T::NT + t], s_cfl[2 * T::NT + t], s_cfl[3 * T::NT + t],
-                        INFINITY};
-        cflf_reduce(p, c, sc, MAG ? sc->dd2min : INFINITY);
+        cflf_reduce(p, CflFast{cfl_acc.mach, cfl_acc.q1, cfl_acc.s2, cfl_acc.s3, INFINITY}, sc,
+                    MAG ? sc->dd2min : INFINITY);
     }
 }
 
