@@ -106,8 +106,9 @@ def accuracy(npx=256, npy=256, npz=128, steps=1000, fixed=True):
             "ms_step_fp64": r["fp64"]["ms_step_events"], "ms_step_mixed": r["mixed"]["ms_step_events"]}
 
 
-def ensemble(pkg, nbox=64, steps=100, precision="mixed"):
-    """The random 64-box ensemble: autotuned tiling, graph-replayed steps,
+def ensemble(pkg, nbox=64, steps=100, precision="mixed", tiling=None):
+    """The random 64-box ensemble: autotuned tiling (or the given
+    (tile_y, kdepth, grid), with no autotuning), graph-replayed steps,
     against the same boxes stepped one at a time through the single-box
     library."""
     specs = wl().ensemble_specs(nbox)
@@ -115,8 +116,12 @@ def ensemble(pkg, nbox=64, steps=100, precision="mixed"):
     with pkg.Ensemble([m for m, _ in boxes], precision=precision) as ens:
         for b, (_, s) in enumerate(boxes):
             ens.upload(b, s)
+        report = None
+        if tiling is not None:
+            ens.set_tiling(*tiling)
         ens.steps(2)
-        report = ens.autotune(nsteps=3)
+        if tiling is None:
+            report = ens.autotune(nsteps=3)
         ms = ens.time_steps(steps)
         zones = ens.zones()
         tiling = ens.tiling()
@@ -130,8 +135,9 @@ def ensemble(pkg, nbox=64, steps=100, precision="mixed"):
             "ms_per_step": ms, "value": zones / (ms * 1e-3), "unit": "zone-updates/s",
             "hbm_frac_step": zones / (ms * 1e-3) * BYTES_PER_ZONE_STEP[precision] / HBM_PEAK,
             "tiling": {"tile_y": tiling[0], "kdepth": tiling[1], "grid": tiling[2]},
-            "autotune_candidates": len(report),
-            "autotune_ms_range": [min(r["ms_step"] for r in report), max(r["ms_step"] for r in report)],
+            "autotune_candidates": len(report) if report else 0,
+            "autotune_ms_range": [min(r["ms_step"] for r in report), max(r["ms_step"] for r in report)]
+                                 if report else None,
             "ms_per_step_boxes_one_at_a_time": ms_single, "speedup_vs_one_at_a_time": ms_single / ms}
 
 
@@ -180,6 +186,8 @@ def main():
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--accuracy-steps", type=int, default=1000, help="0 skips the mixed-vs-FP64 error run")
     ap.add_argument("--ensemble-boxes", type=int, default=64, help="0 skips the batched ensemble run")
+    ap.add_argument("--ensemble-tiling", type=int, nargs=3, metavar=("TILE_Y", "KDEPTH", "GRID"),
+                    help="fixed ensemble tiling instead of autotuning (for profiling one tiling)")
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--ranks", type=int, default=16)
     args = ap.parse_args()
@@ -203,7 +211,7 @@ def main():
     ens = None
     if args.ensemble_boxes > 0:
         log(f"ensemble: {args.ensemble_boxes} boxes")
-        ens = ensemble(pkg, nbox=args.ensemble_boxes)
+        ens = ensemble(pkg, nbox=args.ensemble_boxes, tiling=args.ensemble_tiling)
 
     acc = None
     if args.accuracy_steps > 0 and args.precision == "both":
