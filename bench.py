@@ -108,7 +108,7 @@ def accuracy(npx=256, npy=256, npz=128, steps=1000, fixed=True):
 
 def ensemble(pkg, nbox=64, steps=100, precision="mixed", tiling=None):
     """The random 64-box ensemble: autotuned tiling (or the given
-    (tile_y, kdepth, grid), with no autotuning), graph-replayed steps,
+    (tile_x, tile_y, kdepth, grid), with no autotuning), graph-replayed steps,
     against the same boxes stepped one at a time through the single-box
     library."""
     specs = wl().ensemble_specs(nbox)
@@ -118,13 +118,14 @@ def ensemble(pkg, nbox=64, steps=100, precision="mixed", tiling=None):
             ens.upload(b, s)
         report = None
         if tiling is not None:
-            ens.set_tiling(*tiling)
+            ens.set_tile(*tiling)
         ens.steps(2)
         if tiling is None:
             report = ens.autotune(nsteps=3)
+            log("ensemble autotune: " + json.dumps(report))
         ms = ens.time_steps(steps)
         zones = ens.zones()
-        tiling = ens.tiling()
+        tiling = ens.tile()
     ms_single = 0.0
     for m, s in boxes:
         with pkg.Box(m, precision=precision) as box:
@@ -134,7 +135,7 @@ def ensemble(pkg, nbox=64, steps=100, precision="mixed", tiling=None):
     return {"boxes": nbox, "zones": zones, "precision": precision, "steps": steps,
             "ms_per_step": ms, "value": zones / (ms * 1e-3), "unit": "zone-updates/s",
             "hbm_frac_step": zones / (ms * 1e-3) * BYTES_PER_ZONE_STEP[precision] / HBM_PEAK,
-            "tiling": {"tile_y": tiling[0], "kdepth": tiling[1], "grid": tiling[2]},
+            "tiling": {"tile_x": tiling[0], "tile_y": tiling[1], "kdepth": tiling[2], "grid": tiling[3]},
             "autotune_candidates": len(report) if report else 0,
             "autotune_ms_range": [min(r["ms_step"] for r in report), max(r["ms_step"] for r in report)]
                                  if report else None,
@@ -186,7 +187,7 @@ def main():
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     ap.add_argument("--accuracy-steps", type=int, default=1000, help="0 skips the mixed-vs-FP64 error run")
     ap.add_argument("--ensemble-boxes", type=int, default=64, help="0 skips the batched ensemble run")
-    ap.add_argument("--ensemble-tiling", type=int, nargs=3, metavar=("TILE_Y", "KDEPTH", "GRID"),
+    ap.add_argument("--ensemble-tiling", type=int, nargs=4, metavar=("TILE_X", "TILE_Y", "KDEPTH", "GRID"),
                     help="fixed ensemble tiling instead of autotuning (for profiling one tiling)")
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
     ap.add_argument("--ranks", type=int, default=16)

@@ -8,8 +8,8 @@
  * launches: three persistent stage launches over a work table that covers
  * every box, three boundary-plane launches and one per-box DT/clock update.
  * Repeated steps are replayed from a captured HIP graph.  The tile shape,
- * z-chunk depth and persistent grid size come from mhd_ens_autotune or
- * mhd_ens_set_tiling.
+ * z-chunk depth and persistent grid size come from mhd_ens_autotune,
+ * mhd_ens_set_tiling or mhd_ens_set_tile.
  *
  * Every box advances exactly as the single-box library (rast3dmhd.h) would
  * advance it alone: the same kernel code per point, and reductions that
@@ -58,14 +58,22 @@ int mhd_ens_steps(mhd_ens_t *e, int n, int use_graph, double *dt, double *umach)
 
 /* Tiling of the batched launches: tile_y (4 or 8 rows of 64 columns),
  * kdepth (planes per z chunk; 0 = whole column) and grid (persistent
- * workgroups; 0 = one workgroup per work item). */
+ * workgroups; 0 = one workgroup per work item).  get_tiling reports the
+ * rows of the current tile whatever its width. */
 int mhd_ens_set_tiling(mhd_ens_t *e, int tile_y, int kdepth, int grid);
 int mhd_ens_get_tiling(const mhd_ens_t *e, int *tile_y, int *kdepth, int *grid);
+
+/* The same with the tile width: tile_x x tile_y is 64x4, 64x8, or (mixed
+ * form only) 32x8, which leaves fewer lanes idle on boxes whose rows are
+ * not a multiple of 64 points.  The results do not depend on the tiling. */
+int mhd_ens_set_tile(mhd_ens_t *e, int tile_x, int tile_y, int kdepth, int grid);
+int mhd_ens_get_tile(const mhd_ens_t *e, int *tile_x, int *tile_y, int *kdepth, int *grid);
 
 /* Times every candidate tiling over nsteps steps (HIP events, direct
  * launches), keeps the fastest, and restores the state and clocks of every
  * box to what they were before the call.  report (may be NULL) receives a
- * JSON list of {tile_y, kdepth, grid, ms_step} for the candidates. */
+ * JSON list of {tile_x, tile_y, kdepth, grid, ms_step} for the candidates
+ * (tile shapes as in mhd_ens_set_tile). */
 int mhd_ens_autotune(mhd_ens_t *e, int nsteps, char *report, int report_len);
 
 /* Mean milliseconds per ensemble step over n steps (HIP events on the

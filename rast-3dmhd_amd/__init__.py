@@ -87,6 +87,8 @@ def load(path: str | None = None):
     lib.mhd_ens_steps.argtypes = [V, I, I, _P, _P]
     lib.mhd_ens_set_tiling.argtypes = [V, I, I, I]
     lib.mhd_ens_get_tiling.argtypes = [V, C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+    lib.mhd_ens_set_tile.argtypes = [V, I, I, I, I]
+    lib.mhd_ens_get_tile.argtypes = [V, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)]
     lib.mhd_ens_autotune.argtypes = [V, I, C.c_char_p, I]
     lib.mhd_ens_time_steps.argtypes = [V, I, _P]
     lib.mhd_ens_zones.argtypes = [V]
@@ -123,7 +125,8 @@ def load(path: str | None = None):
         getattr(lib, fn).restype = C.c_int
     for fn in ("mhd_ens_create", "mhd_ens_set_metrics", "mhd_ens_set_background", "mhd_ens_upload",
                "mhd_ens_download", "mhd_ens_set_clock", "mhd_ens_get_clock", "mhd_ens_set_fixed_dt", "mhd_ens_steps", "mhd_ens_set_tiling",
-               "mhd_ens_get_tiling", "mhd_ens_autotune", "mhd_ens_time_steps"):
+               "mhd_ens_get_tiling", "mhd_ens_set_tile", "mhd_ens_get_tile", "mhd_ens_autotune",
+               "mhd_ens_time_steps"):
         getattr(lib, fn).restype = C.c_int
     for fn in ("mhd_create", "mhd_set_metrics", "mhd_set_background", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
                "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps", "mhd_set_fixed_dt", "mhd_box_filter",
@@ -392,6 +395,16 @@ class Ensemble:
     def tiling(self):
         v = [C.c_int() for _ in range(3)]
         self._check(self.lib.mhd_ens_get_tiling(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_tile(self, tile_x: int, tile_y: int, kdepth: int, grid: int):
+        """Tiling with the tile width: 64x4, 64x8 or (mixed form) 32x8."""
+        self._check(self.lib.mhd_ens_set_tile(self.h, int(tile_x), int(tile_y), int(kdepth), int(grid)))
+
+    def tile(self):
+        """(tile_x, tile_y, kdepth, grid) of the current tiling."""
+        v = [C.c_int() for _ in range(4)]
+        self._check(self.lib.mhd_ens_get_tile(self.h, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
     def autotune(self, nsteps: int = 5):

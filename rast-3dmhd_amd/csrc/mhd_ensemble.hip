@@ -59,7 +59,7 @@ struct mhd_ens {
     double *stage = nullptr;         // one padded field of the largest box
     size_t stage_bytes = 0;
     // tiling
-    int tile_y = 4, kdepth = 16, grid = 0;
+    int tile_x = 64, tile_y = 4, kdepth = 16, grid = 0;
     StageItem *ditems = nullptr;
     int nitems = 0;
     ColumnItem *dcols = nullptr;
@@ -114,21 +114,21 @@ static int check_box(mhd_ens_t *e, int b, const char *who) {
     return MHD_OK;
 }
 
-// Work tables for the current tiling: StageItems box-major (z chunk, row,
-// x tile), ColumnItems 256 columns each.
+// Work tables for the current tiling: StageItems (tile_x x tile_y tiles)
+// box-major (z chunk, row, x tile), ColumnItems 256 columns each.
 static int build_tables(mhd_ens_t *e) {
     std::vector<StageItem> items;
     std::vector<ColumnItem> cols;
     for (int b = 0; b < e->nbox; ++b) {
         const Params &p = e->box[b].p;
         const int kd = e->kdepth > 0 ? e->kdepth : p.npz;
-        const int ntx = (p.npx + 63) / 64, nty = (p.npy + e->tile_y - 1) / e->tile_y;
+        const int ntx = (p.npx + e->tile_x - 1) / e->tile_x, nty = (p.npy + e->tile_y - 1) / e->tile_y;
         for (int k0 = 1; k0 <= p.npz; k0 += kd)
             for (int ty = 0; ty < nty; ++ty)
                 for (int tx = 0; tx < ntx; ++tx) {
                     StageItem it;
                     it.box = b;
-                    it.i0 = tx * 64;
+                    it.i0 = tx * e->tile_x;
                     it.j0 = ty * e->tile_y;
                     it.kbeg = k0;
                     it.kend = k0 + kd - 1 < p.npz ? k0 + kd - 1 : p.npz;
@@ -409,7 +409,8 @@ static hipError_t enqueue_step(mhd_ens_t *e, int a) {
             r = launch_stage_batch(st, e->tile_y, e->lmag, e->dbox, e->ditems, e->nitems, grid, a, e->stream);
             if (r == hipSuccess) r = launch_bcon_batch(e->lmag, e->dbox, e->dcols, e->ncols, a, st == 3, e->stream);
         } else {
-            r = launch_stage_batch_mixed(st, e->tile_y, e->lmag, e->dbox, e->ditems, e->nitems, grid, a, e->stream);
+            r = launch_stage_batch_mixed(st, e->tile_x, e->tile_y, e->lmag, e->dbox, e->ditems, e->nitems, grid, a,
+                                         e->stream);
             if (r == hipSuccess)
                 r = launch_bcon_batch_mixed(e->lmag, e->dbox, e->dcols, e->ncols, a, st == 3, e->stream);
         }
@@ -515,25 +516,41 @@ extern "C" int mhd_ens_steps(mhd_ens_t *e, int n, int use_graph, double *dt, dou
     return MHD_OK;
 }
 
-extern "C" int mhd_ens_set_tiling(mhd_ens_t *e, int tile_y, int kdepth, int grid) {
-    if (!e) return efail(nullptr, MHD_ERR_ARG, "mhd_ens_set_tiling: null handle");
-    if ((tile_y != 4 && tile_y != 8) || kdepth < 0 || grid < 0)
-        return efail(e, MHD_ERR_ARG, "mhd_ens_set_tiling: tile_y %d (4 or 8), kdepth %d, grid %d", tile_y, kdepth,
-                     grid);
+extern "C" int mhd_ens_set_tile(mhd_ens_t *e, int tile_x, int tile_y, int kdepth, int grid) {
+    if (!e) return efail(nullptr, MHD_ERR_ARG, "mhd_ens_set_tile: null handle");
+    const bool shape_ok = (tile_x == 64 && (tile_y == 4 || tile_y == 8)) || (tile_x == 32 && tile_y == 8 && e->mixed);
+    if (!shape_ok || kdepth < 0 || grid < 0)
+        return efail(e, MHD_ERR_ARG,
+                     "mhd_ens_set_tile: tile %dx%d (64x4, 64x8, or 32x8 in the mixed form), kdepth %d, grid %d",
+                     tile_x, tile_y, kdepth, grid);
     EHIP(e, hipSetDevice(e->dev));
     EHIP(e, hipStreamSynchronize(e->stream));
+    e->tile_x = tile_x;
     e->tile_y = tile_y;
     e->kdepth = kdepth;
     e->grid = grid;
     return build_tables(e);
 }
 
-extern "C" int mhd_ens_get_tiling(const mhd_ens_t *e, int *tile_y, int *kdepth, int *grid) {
+extern "C" int mhd_ens_get_tile(const mhd_ens_t *e, int *tile_x, int *tile_y, int *kdepth, int *grid) {
     if (!e) return MHD_ERR_ARG;
+    if (tile_x) *tile_x = e->tile_x;
     if (tile_y) *tile_y = e->tile_y;
     if (kdepth) *kdepth = e->kdepth;
     if (grid) *grid = e->grid;
     return MHD_OK;
+}
+
+extern "C" int mhd_ens_set_tiling(mhd_ens_t *e, int tile_y, int kdepth, int grid) {
+    if (!e) return efail(nullptr, MHD_ERR_ARG, "mhd_ens_set_tiling: null handle");
+    if ((tile_y != 4 && tile_y != 8) || kdepth < 0 || grid < 0)
+        return efail(e, MHD_ERR_ARG, "mhd_ens_set_tiling: tile_y %d (4 or 8), kdepth %d, grid %d", tile_y, kdepth,
+                     grid);
+    return mhd_ens_set_tile(e, 64, tile_y, kdepth, grid);
+}
+
+extern "C" int mhd_ens_get_tiling(const mhd_ens_t *e, int *tile_y, int *kdepth, int *grid) {
+    return mhd_ens_get_tile(e, nullptr, tile_y, kdepth, grid);
 }
 
 // Mean ms per step over n steps queued with or without graphs.
@@ -613,18 +630,19 @@ extern "C" int mhd_ens_autotune(mhd_ens_t *e, int nsteps, char *report, int repo
         return x;
     };
     r = copy_state(true);
-    const int t0y = e->tile_y, t0k = e->kdepth, t0g = e->grid;
-    const int tys[2] = {4, 8};
+    const int t0x = e->tile_x, t0y = e->tile_y, t0k = e->kdepth, t0g = e->grid;
+    const int txs[3] = {64, 64, 32}, tys[3] = {4, 8, 8};  // 32x8 in the mixed form only
+    const int nshape = e->mixed ? 3 : 2;
     const int kds[4] = {8, 16, 32, 0};
     const int per_cu[4] = {0, 1, 2, 4};  // 0: one workgroup per item
     double best = 1e300;
-    int bty = t0y, bkd = t0k, bgr = t0g;
+    int btx = t0x, bty = t0y, bkd = t0k, bgr = t0g;
     std::string rep = "[";
-    for (int a = 0; a < 2 && r == hipSuccess && !rc; ++a)
+    for (int a = 0; a < nshape && r == hipSuccess && !rc; ++a)
         for (int k = 0; k < 4 && r == hipSuccess && !rc; ++k)
             for (int g = 0; g < 4 && r == hipSuccess && !rc; ++g) {
                 const int grid = per_cu[g] * e->cus;
-                rc = mhd_ens_set_tiling(e, tys[a], kds[k], grid);
+                rc = mhd_ens_set_tile(e, txs[a], tys[a], kds[k], grid);
                 if (rc) break;
                 double ms = 0;
                 rc = time_run(e, 1, 0, &ms);  // warm-up (kernel attributes, caches)
@@ -633,11 +651,13 @@ extern "C" int mhd_ens_autotune(mhd_ens_t *e, int nsteps, char *report, int repo
                 r = copy_state(false);
                 e->cur = cur0;
                 char line[160];
-                snprintf(line, sizeof line, "%s{\"tile_y\": %d, \"kdepth\": %d, \"grid\": %d, \"ms_step\": %.5f}",
-                         rep.size() > 1 ? ", " : "", tys[a], kds[k], grid, ms);
+                snprintf(line, sizeof line,
+                         "%s{\"tile_x\": %d, \"tile_y\": %d, \"kdepth\": %d, \"grid\": %d, \"ms_step\": %.5f}",
+                         rep.size() > 1 ? ", " : "", txs[a], tys[a], kds[k], grid, ms);
                 rep += line;
                 if (ms < best) {
                     best = ms;
+                    btx = txs[a];
                     bty = tys[a];
                     bkd = kds[k];
                     bgr = grid;
@@ -649,7 +669,7 @@ extern "C" int mhd_ens_autotune(mhd_ens_t *e, int nsteps, char *report, int repo
     (void)hipFree(ssave);
     if (rc) return rc;
     if (r != hipSuccess) return efail(e, MHD_ERR_HIP, "mhd_ens_autotune: %s", hipGetErrorString(r));
-    rc = mhd_ens_set_tiling(e, bty, bkd, bgr);
+    rc = mhd_ens_set_tile(e, btx, bty, bkd, bgr);
     if (rc) return rc;
     if (report && report_len > 0) {
         strncpy(report, rep.c_str(), (size_t)report_len - 1);

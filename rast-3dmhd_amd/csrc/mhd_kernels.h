@@ -177,8 +177,8 @@ hipError_t launch_pack_mixed(const Params &p, const double *padded, int field, c
 hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const unsigned *rreg, int field,
                                const Background &bg, double *padded, hipStream_t s);
 
-hipError_t launch_stage_batch_mixed(int mode, int tile_y, int lmag, const BoxDev *boxes, const StageItem *items,
-                                    int nitems, int grid, int a, hipStream_t s);
+hipError_t launch_stage_batch_mixed(int mode, int tile_x, int tile_y, int lmag, const BoxDev *boxes,
+                                    const StageItem *items, int nitems, int grid, int a, hipStream_t s);
 hipError_t launch_bcon_batch_mixed(int lmag, const BoxDev *boxes, const ColumnItem *items, int nitems, int a,
                                    bool last, hipStream_t s);
 

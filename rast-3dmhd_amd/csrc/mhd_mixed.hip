@@ -42,11 +42,14 @@ constexpr int MTX = 64;
 // Arithmetic type of the perturbation right-hand side (storage stays FP32).
 typedef float rhs_t;
 
-template <int TY_>
+// Tile of TX columns x TY rows, one point per thread (TX = 64: one row per
+// wavefront; TX = 32: two rows per wavefront, for narrow ensemble boxes).
+template <int TX_, int TY_>
 struct MTile {
+    static constexpr int TX = TX_;
     static constexpr int TY = TY_;
-    static constexpr int NT = MTX * TY;
-    static constexpr int SW = MTX + 2;
+    static constexpr int NT = TX * TY;
+    static constexpr int SW = TX + 2;
     static constexpr int SH = TY + 2;
     static constexpr int SP = SW * SH;
     static constexpr int NHALO = SP - NT;
@@ -55,9 +58,9 @@ struct MTile {
 
 // LDS: raw planes k, k+1 (2 x NF fields) and a three-plane ring of u, v, w,
 // T'.
-template <int TY, bool MAG, int MODE>
+template <int TY, bool MAG, int MODE, int TXW = MTX>
 constexpr int mixed_lds_bytes() {
-    return (2 * (MAG ? 8 : 5) + 12) * MTile<TY>::SP * (int)sizeof(float);
+    return (2 * (MAG ? 8 : 5) + 12) * MTile<TXW, TY>::SP * (int)sizeof(float);
 }
 
 // P' of temperature perturbation tp at density perturbation rh.  The
@@ -165,13 +168,13 @@ __device__ __forceinline__ void load_rawf(const float *__restrict__ q, long long
 #define PLB(F, K) uload(&bg.planes[K].F, 0)
 #define PLP(F, K) uload(&bg.planes[K].F, 0)
 
-template <int TY, bool MAG, int MODE>
+template <int TY, bool MAG, int MODE, int TXW = MTX>
 __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, const Background &bg,
                                              const float *__restrict__ hin, float *__restrict__ hout,
                                              float *__restrict__ dr, unsigned *__restrict__ rr,
                                              DevScalars *__restrict__ sc, int i0, int j0, int kbeg, int kend,
                                              float *ldsf) {
-    using T = MTile<TY>;
+    using T = MTile<TXW, TY>;
     constexpr int SW = T::SW, SP = T::SP;
     constexpr int NF = MAG ? 8 : 5;
     float *const s_cur = ldsf;
@@ -181,7 +184,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
     const int npx = p.npx, npy = p.npy, npz = p.npz;
     const long long fs = p.fsize, pl = p.plane;
 
-    const int olx = t % MTX + 1, oly = t / MTX + 1;
+    const int olx = t % TXW + 1, oly = t / TXW + 1;
     const int opt = oly * SW + olx;
     const int gi = i0 + olx - 1, gj = j0 + oly - 1;
     const bool valid = (gi < npx) && (gj < npy);
@@ -537,9 +540,10 @@ __global__ __launch_bounds__(MTX *TY, 3) void k_stage_m(const Params p, const Me
     stage_tile_m<TY, MAG, MODE>(p, m, bg, hin, hout, dr, rr, sc, txb * MTX, tyb * TY, kbeg, kend, ldsf);
 }
 
-// Ensemble: persistent grid over the work items of all boxes (static stride).
-template <int TY, bool MAG, int MODE>
-__global__ __launch_bounds__(MTX *TY, 3) void k_stage_m_batch(const BoxDev *__restrict__ boxes,
+// Ensemble: persistent grid over the work items of all boxes (static stride);
+// items are TXW x TY tiles.
+template <int TY, bool MAG, int MODE, int TXW>
+__global__ __launch_bounds__(TXW *TY, 3) void k_stage_m_batch(const BoxDev *__restrict__ boxes,
                                                            const StageItem *__restrict__ items, int nitems, int a) {
     extern __shared__ float ldsf[];
     const int slot = xcd_remap(blockIdx.x, gridDim.x);
@@ -547,7 +551,7 @@ __global__ __launch_bounds__(MTX *TY, 3) void k_stage_m_batch(const BoxDev *__re
         const StageItem it = items[w];
         const BoxDev &B = boxes[it.box];
         __syncthreads();  // the previous item's last LDS reads are done
-        stage_tile_m<TY, MAG, MODE>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.rr, B.sc, it.i0, it.j0, it.kbeg,
+        stage_tile_m<TY, MAG, MODE, TXW>(B.p, B.m, B.bg, B.h[a], B.h[1 - a], B.dr, B.rr, B.sc, it.i0, it.j0, it.kbeg,
                                     it.kend, ldsf);
     }
 }
@@ -858,39 +862,42 @@ hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const unsigne
     return hipGetLastError();
 }
 
-template <int TY, bool MAG, int MODE>
+template <int TY, bool MAG, int MODE, int TXW>
 static hipError_t launch_mb(const BoxDev *boxes, const StageItem *items, int nitems, int grid, int a,
                             hipStream_t s) {
-    constexpr int lds = mixed_lds_bytes<TY, MAG, MODE>();
+    constexpr int lds = mixed_lds_bytes<TY, MAG, MODE, TXW>();
     static bool attr = false;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_m_batch<TY, MAG, MODE>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_m_batch<TY, MAG, MODE, TXW>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL((k_stage_m_batch<TY, MAG, MODE>), dim3(grid), dim3(MTX * TY), lds, s, boxes, items, nitems,
-                       a);
+    hipLaunchKernelGGL((k_stage_m_batch<TY, MAG, MODE, TXW>), dim3(grid), dim3(TXW * TY), lds, s, boxes, items,
+                       nitems, a);
     return hipGetLastError();
 }
 
-template <int TY>
+template <int TY, int TXW>
 static hipError_t launch_mb_ty(int mode, int lmag, const BoxDev *boxes, const StageItem *items, int nitems,
                                int grid, int a, hipStream_t s) {
     if (lmag) {
-        if (mode == 1) return launch_mb<TY, true, 1>(boxes, items, nitems, grid, a, s);
-        if (mode == 2) return launch_mb<TY, true, 2>(boxes, items, nitems, grid, a, s);
-        return launch_mb<TY, true, 3>(boxes, items, nitems, grid, a, s);
+        if (mode == 1) return launch_mb<TY, true, 1, TXW>(boxes, items, nitems, grid, a, s);
+        if (mode == 2) return launch_mb<TY, true, 2, TXW>(boxes, items, nitems, grid, a, s);
+        return launch_mb<TY, true, 3, TXW>(boxes, items, nitems, grid, a, s);
     }
-    if (mode == 1) return launch_mb<TY, false, 1>(boxes, items, nitems, grid, a, s);
-    if (mode == 2) return launch_mb<TY, false, 2>(boxes, items, nitems, grid, a, s);
-    return launch_mb<TY, false, 3>(boxes, items, nitems, grid, a, s);
+    if (mode == 1) return launch_mb<TY, false, 1, TXW>(boxes, items, nitems, grid, a, s);
+    if (mode == 2) return launch_mb<TY, false, 2, TXW>(boxes, items, nitems, grid, a, s);
+    return launch_mb<TY, false, 3, TXW>(boxes, items, nitems, grid, a, s);
 }
 
-hipError_t launch_stage_batch_mixed(int mode, int tile_y, int lmag, const BoxDev *boxes, const StageItem *items,
-                                    int nitems, int grid, int a, hipStream_t s) {
-    if (tile_y == 8) return launch_mb_ty<8>(mode, lmag, boxes, items, nitems, grid, a, s);
-    return launch_mb_ty<4>(mode, lmag, boxes, items, nitems, grid, a, s);
+// Tile shapes built for the batched launches: 64x4, 64x8 and 32x8 (the
+// caller validates the shape).
+hipError_t launch_stage_batch_mixed(int mode, int tile_x, int tile_y, int lmag, const BoxDev *boxes,
+                                    const StageItem *items, int nitems, int grid, int a, hipStream_t s) {
+    if (tile_x == 32) return launch_mb_ty<8, 32>(mode, lmag, boxes, items, nitems, grid, a, s);
+    if (tile_y == 8) return launch_mb_ty<8, MTX>(mode, lmag, boxes, items, nitems, grid, a, s);
+    return launch_mb_ty<4, MTX>(mode, lmag, boxes, items, nitems, grid, a, s);
 }
 
 hipError_t launch_bcon_batch_mixed(int lmag, const BoxDev *boxes, const ColumnItem *items, int nitems, int a,

@@ -66,14 +66,24 @@ def test_ensemble_bitwise_equal_single_box(precision):
         check_equal(ens, refs)
 
 
-@pytest.mark.parametrize("tiling", [(4, 8, 0), (8, 0, 256), (4, 16, 512), (8, 32, 1024)])
+# (tile_x, tile_y, kdepth, grid); the 64-wide shapes through the 3-value call
+TILINGS = [(64, 4, 8, 0), (64, 8, 0, 256), (64, 4, 16, 512), (64, 8, 32, 1024), (32, 8, 16, 0), (32, 8, 0, 512)]
+# odd sizes: partial tiles in x and y for every tile shape
+ODD = dict(npx=45, npy=37, npz=35, ra=5e4, ta=1e3, contrast=8.0, seed=106)
+
+
+@pytest.mark.parametrize("tiling", TILINGS)
 def test_ensemble_tiling_invariance(tiling):
     say("case ensemble tiling", tiling)
-    bx = boxes(SPECS[:3])
+    bx = boxes(SPECS[:3] + [ODD])
     refs = [alone(m, s, "mixed", 4) for m, s in bx]
     with H.pkg().Ensemble([m for m, _ in bx], precision="mixed") as ens:
-        ens.set_tiling(*tiling)
-        assert ens.tiling() == tiling
+        if tiling[0] == 64:
+            ens.set_tiling(*tiling[1:])
+            assert ens.tiling() == tiling[1:]
+        else:
+            ens.set_tile(*tiling)
+        assert ens.tile() == tiling
         for b, (_, s) in enumerate(bx):
             ens.upload(b, s)
         ens.steps(2, graph=False)
@@ -96,6 +106,28 @@ def test_ensemble_autotune_keeps_state():
         assert ens.tiling() == (best["tile_y"], best["kdepth"], best["grid"])
         ens.steps(3)
         check_equal(ens, refs)
+
+
+def test_ensemble_autotune_mixed_tile_shapes():
+    """The mixed form also tries 32x8 tiles: 3 shapes x 4 depths x 4 grids."""
+    say("case ensemble autotune mixed")
+    bx = boxes(SPECS[:2] + [ODD])
+    refs = [alone(m, s, "mixed", 3) for m, s in bx]
+    with H.pkg().Ensemble([m for m, _ in bx], precision="mixed") as ens:
+        for b, (_, s) in enumerate(bx):
+            ens.upload(b, s)
+        ens.steps(1)
+        report = ens.autotune(nsteps=1)
+        assert len(report) == 48
+        assert sum(r["tile_x"] == 32 for r in report) == 16
+        best = min(report, key=lambda r: r["ms_step"])
+        say("autotune", ens.tile(), best["ms_step"])
+        assert ens.tile() == (best["tile_x"], best["tile_y"], best["kdepth"], best["grid"])
+        ens.steps(2)
+        check_equal(ens, refs)
+        with pytest.raises(H.pkg().MhdError) as e:
+            ens.set_tile(32, 4, 8, 0)
+        assert e.value.code == H.pkg().MHD_ERR_ARG
 
 
 def test_ensemble_reupload_one_box():
@@ -124,6 +156,9 @@ def test_ensemble_errors():
         assert e.value.code == pkg.MHD_ERR_STATE
         with pytest.raises(pkg.MhdError) as e:
             ens.set_tiling(5, 8, 0)
+        assert e.value.code == pkg.MHD_ERR_ARG
+        with pytest.raises(pkg.MhdError) as e:
+            ens.set_tile(32, 8, 8, 0)  # 32-wide tiles are built for the mixed form only
         assert e.value.code == pkg.MHD_ERR_ARG
         # the C entry point checks the box index itself
         assert ens.lib.mhd_ens_download(ens.h, 7, *([None] * 8)) == pkg.MHD_ERR_ARG
