@@ -52,12 +52,20 @@ __device__ __forceinline__ void settle(X x) {
 
 // Stored values of the mixed form (mhd_mixed.hip) as other kernels read them.
 
+// 1/x of the FP32 perturbation arithmetic by the hardware reciprocal
+// (v_rcp_f32, within 1 ulp) instead of the correctly rounded division
+// sequence (about ten instructions).  The mixed form is judged by its error
+// against the FP64 form, where one ulp of 1/rho is far below the FP32
+// rounding of the stored values; every use (stencil, packing, unpacking,
+// boundary planes) goes through this function, so they stay consistent.
+__device__ __forceinline__ float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // T' represented by a stored (rho', P') pair: (P' - T0 rho') / (rho0 + rho')
 // in FP32.  Every kernel that needs T' of a stored pair calls this function
 // (the stencil's ring included), so the value the residual compensates
 // against is bit-identical to the value the right-hand side sees.
 __device__ __forceinline__ float trep(float r0, float t0, float rh, float pp) {
-    const float ri = 1.0f / (r0 + rh);
+    const float ri = rcp32(r0 + rh);
     return (pp - t0 * rh) * ri;
 }
 

@@ -209,7 +209,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         for (int f = 0; f < NF; ++f) cur(k, f, pt) = r.v[f];
     };
     auto put_ring = [&](int k, int pt, const RawF &r) {
-        const float ri = 1.0f / (PLB(ro32, k) + r.v[3]);
+        const float ri = rcp32(PLB(ro32, k) + r.v[3]);
         ring(k, 0, pt) = r.v[0] * ri;
         ring(k, 1, pt) = r.v[1] * ri;
         ring(k, 2, pt) = r.v[2] * ri;
@@ -303,7 +303,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             }
             const rhs_t rh0 = q0[3], th0 = RNG(k, 3, 0);
             const rhs_t tf0 = t0 + th0;                    // full temperature
-            const rhs_t rinv = rhs_t(1) / (r0 + rh0);          // 1/rho
+            const rhs_t rinv = rcp32(r0 + rh0);                // 1/rho
             const rhs_t u0 = RNG(k, 0, 0), v0 = RNG(k, 1, 0), w0 = RNG(k, 2, 0);
             const rhs_t um = RNG(k - 1, 0, 0), vm = RNG(k - 1, 1, 0), wm = RNG(k - 1, 2, 0);
             const rhs_t up = RNG(k + 1, 0, 0), vp = RNG(k + 1, 1, 0), wp = RNG(k + 1, 2, 0);
@@ -323,7 +323,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 const int kb = top ? k + 3 : k - 3;
                 const long long oa = (long long)ka * pl + ooff, ob = (long long)kb * pl + ooff;
                 rwk2 = hin[2 * fs + oa];
-                wk2 = rwk2 * (rhs_t(1) / ((rhs_t)PLB(ro32, ka) + (rhs_t)hin[3 * fs + oa]));
+                wk2 = rwk2 * rcp32((rhs_t)PLB(ro32, ka) + (rhs_t)hin[3 * fs + oa]);
                 if (MAG) {
                     bxk2 = hin[5 * fs + oa];
                     byk2 = hin[6 * fs + oa];
