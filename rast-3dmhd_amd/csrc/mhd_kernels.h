@@ -117,6 +117,20 @@ __host__ __device__ inline unsigned long long dkey(double v) {
     unsigned long long b = __builtin_bit_cast(unsigned long long, v);
     return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
 }
+// Updates of a reduction key.  Between its reset (by an earlier launch or
+// the host) and its read (by a later launch) a key only moves one way, so
+// a workgroup whose value cannot move it, by a read of the key at device
+// scope (coherent across the XCDs' L2 caches), skips the atomic: a stale
+// read can only cost an atomic that was not needed, never skip one that
+// was.  Without the check every workgroup of a launch queues an atomic on
+// the same cache line; for the 1024 short workgroups of a boundary-plane
+// launch that queue was three quarters of the launch.
+__device__ inline void key_raise(unsigned long long *k, unsigned long long v) {
+    if (v > __hip_atomic_load(k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(k, v);
+}
+__device__ inline void key_lower(unsigned long long *k, unsigned long long v) {
+    if (v < __hip_atomic_load(k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(k, v);
+}
 __host__ __device__ inline double dunkey(unsigned long long k) {
     unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
     return __builtin_bit_cast(double, b);

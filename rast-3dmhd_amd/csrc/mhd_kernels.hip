@@ -123,11 +123,11 @@ __device__ void cfl_reduce(double mach, double w1, double w2, double w3, double 
             k3 = red[v][3] < k3 ? red[v][3] : k3;
             k4 = red[v][4] < k4 ? red[v][4] : k4;
         }
-        atomicMax(&sc->key_mach, km);
-        atomicMin(&sc->key_min[0], k1);
-        atomicMin(&sc->key_min[1], k2);
-        atomicMin(&sc->key_min[2], k3);
-        if (lmag) atomicMin(&sc->key_min[3], k4);
+        key_raise(&sc->key_mach, km);
+        key_lower(&sc->key_min[0], k1);
+        key_lower(&sc->key_min[1], k2);
+        key_lower(&sc->key_min[2], k3);
+        if (lmag) key_lower(&sc->key_min[3], k4);
     }
 }
 

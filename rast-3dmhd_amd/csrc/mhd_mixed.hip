@@ -141,11 +141,11 @@ __device__ void cflf_reduce(const Params &p, CflFast c, DevScalars *__restrict__
         }
         c.d2 = nmin(c.d2, d2extra);
         const double w[4] = {1.0 / sqrt(c.q1), 0.5 * p.repr * p.cv * c.s2, 0.375 * p.re * c.s3, 0.5 * p.rm * c.d2};
-        atomicMax(&sc->key_mach, dkey(c.mach));
+        key_raise(&sc->key_mach, dkey(c.mach));
         for (int n = 0; n < 4; ++n) {
             if (n == 3 && !p.lmag) break;
             // a NaN limit becomes -NaN for the min reduction
-            atomicMin(&sc->key_min[n], (w[n] == w[n]) ? dkey(w[n]) : 0ull);
+            key_lower(&sc->key_min[n], (w[n] == w[n]) ? dkey(w[n]) : 0ull);
         }
     }
 }
