@@ -193,8 +193,8 @@ def main():
     ap.add_argument("--ranks", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs the whole box resident on one GPU; "
-              "multi-GPU runs are not part of it (see DESIGN.md, 'More than one GPU').", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus}: the whole box runs resident on one GPU; runs on more than "
+              "one GPU are not part of this implementation (use --gpus 1).", file=sys.stderr)
         sys.exit(2)
 
     pkg = importlib.import_module("rast-3dmhd_amd")

@@ -1,0 +1,31 @@
+"""bench.py's command line on the CPU: the one-GPU contract and the options
+the committed profiles use (no GPU work is started)."""
+
+import os
+import subprocess
+import sys
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def run(*args):
+    return subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), *args], capture_output=True, text=True,
+                          timeout=120)
+
+
+def test_more_than_one_gpu_exits_with_a_message():
+    r = run("--gpus", "2")
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_help_lists_the_profiling_options():
+    r = run("--help")
+    assert r.returncode == 0
+    for opt in ("--precision", "--ensemble-boxes", "--accuracy-steps", "--cpu-baseline", "--ensemble-tiling"):
+        assert opt in r.stdout
+
+
+def test_ensemble_tiling_takes_four_values():
+    r = run("--gpus", "2", "--ensemble-tiling", "32", "8", "16")
+    assert r.returncode == 2 and "expected 4 arguments" in r.stderr
