@@ -55,6 +55,30 @@ def _run(cmd, cwd):
     return r
 
 
+# Statements of the reference main program that the harness runs as they
+# are: the COMMON set-up (constants, SETUP, IPAR/PAR to COMMON) and the
+# conductivity profile.  (1-based line range of 3dmhd.f, expected first and
+# last statement.)
+DRIVER_BLOCKS = {
+    "drvset1.inc": (105, 240, "IXC=IXCON", "endif"),
+    "drvset2.inc": (269, 279, "IF (.NOT.LREM) THEN", "ENDIF"),
+}
+
+
+def write_driver_blocks(d: str) -> None:
+    """Extract DRIVER_BLOCKS from the reference's 3dmhd.f into the build
+    directory d (git-ignored): the harness INCLUDEs them, so the reference's
+    own set-up statements are never kept in this repository."""
+    with open(os.path.join(REF_SRC, "3dmhd.f")) as f:
+        lines = f.read().split("\n")
+    for name, (a, b, first, last) in DRIVER_BLOCKS.items():
+        block = lines[a - 1:b]
+        if block[0].strip() != first or block[-1].strip() != last:
+            raise RuntimeError(f"3dmhd.f:{a}-{b} is not the expected block for {name}")
+        with open(os.path.join(d, name), "w") as f:
+            f.write("\n".join(block) + "\n")
+
+
 def build(name: str, cfg: dict | None = None, main_exe: bool = False) -> str:
     """Compile the reference for one box; returns the build directory."""
     cfg = cfg or refconfigs.CONFIGS[name]
@@ -62,6 +86,7 @@ def build(name: str, cfg: dict | None = None, main_exe: bool = False) -> str:
     os.makedirs(d, exist_ok=True)
     mkparam.write_config(d, cfg["params"], par=cfg["par"], ipar=cfg.get("ipar"),
                          env_steps=main_exe)
+    write_driver_blocks(d)
     links = {"3dmhdsub.f": os.path.join(REF_SRC, "3dmhdsub.f"),
              "refharness.f": os.path.join(HERE, "refharness.f")}
     if main_exe:

@@ -74,94 +74,16 @@ C
       READ(30,*) AMPU,AMPBP,ISEED
       CLOSE(30)
 C
-C     COMMON set-up as in the reference main program.
-      IXC=IXCON
-      IYC=IYCON
-      IZC=IZCON
-      ITC=ITCON
-      IBC=IBCON
-      SF=SFF
-      C13=1.0E00/3.0E00
-      C23=2.0E00*C13
-      C43=4.0E00*C13
-      GAM1=8.0E00/15.0E00
-      GAM2=5.0E00/12.0E00
-      GAM3=3.0E00/4.0E00
-      ZETA1=-17.0E00/60.0E00
-      ZETA2=-5.0E00/12.0E00
-      HX=0.5E00*FLOAT(NPX-1)
-      H2X=FLOAT(NPX-1)*FLOAT(NPX-1)
-      HY=0.5E00*FLOAT(NPY-1)
-      H2Y=FLOAT(NPY-1)*FLOAT(NPY-1)
-      HZ=0.5E00*FLOAT(NPZ-1)
-      H2Z=FLOAT(NPZ-1)*FLOAT(NPZ-1)
-      CALL SETUP(FINP,FOUT,IPAR,PAR)
-      NTOTAL=IPAR(03)
-      NSTEP0=IPAR(04)
-      NIT=0
-      RE=PAR(01)
-      PR=PAR(02)
-      THETA=PAR(03)
-      GRAV=PAR(04)
-      RY=PAR(05)
-      ANG=PAR(06)
-      RM=PAR(07)
-      BETA=PAR(08)
-      PZP=PAR(09)
-      SIGMA=PAR(10)
-      POLYS=PAR(11)
-      TP=PAR(12)
-      TC=PAR(13)
-      XP=PAR(14)
-      YP=PAR(15)
-      ZP=PAR(16)
-      XMAX=PAR(17)
-      YMAX=PAR(18)
-      ZMAX=PAR(19)
-      AMPT=PAR(20)
-      AMPB=PAR(21)
-      BFH=PAR(22)
-      BZP=PAR(23)
-      QFH=PAR(53)
-      GAMMA=PAR(54)
-      HH=PAR(55)
-      TSTART=PAR(58)
-      TOFF=PAR(59)
-      RLAX=PAR(60)
-      IF (LROT) THEN
-         OMX=SIN(ANG)/RY
-         OMZ=COS(ANG)/RY
-      ELSE
-         OMX=0.0E00
-         OMZ=0.0E00
-      ENDIF
-      REPR=RE*PR
-      ORE=1.0E00/RE
-      IF (GRAV.EQ.0.0E00) THEN
-         RKAPST=1.0E00
-      ELSE
-         RKAPST=(POLYS+1.0E00)*THETA/GRAV
-      ENDIF
-      IF (LMAG) THEN
-         ORM=1.0E00/RM
-         OBETA=2.0E00/BETA
-      ELSE
-         ORM=0.0
-         OBETA=0.0
-      ENDIF
+C     COMMON set-up of the reference main program.  The statements are
+C     the reference's own (3dmhd.f:105-240: constants, SETUP, IPAR/PAR to
+C     COMMON; 3dmhd.f:269-279: RKAPA, DKAPA), extracted from the reference
+C     source at build time by build_ref.py into the two include files.
+      INCLUDE 'drvset1.inc'
       TIMC=0.0E00
       CALL XJACOBI
       CALL YJACOBI
       CALL ZJACOBI
-      IF (PZP.EQ.0.0E00) THEN
-         RKAPA=1.0E00
-         DKAPA=0.0E00
-      ELSE
-         RKAPA=1.0E00
-     2        +(RKAPST-1.0E00)/2.0E00*(1.0E00+TANH((ZEE-PZP)/SIGMA))
-         DKAPA=(RKAPST-1.0E00)/2.0E00/SIGMA
-     2        /COSH((ZEE-PZP)/SIGMA)**2.0D00
-      ENDIF
+      INCLUDE 'drvset2.inc'
 C     A first STATIC call without T noise or B layer yields the bare
 C     polytrope profiles T(z), R(z) (one column), which let tests rebuild
 C     the perturbed initial state exactly; the second call is the real one.

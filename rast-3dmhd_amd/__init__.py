@@ -155,6 +155,23 @@ def _ptr(a):
     return None if a is None else a.ctypes.data_as(_P)
 
 
+def metric_arrays(meta: dict, desc: MhdDesc) -> list:
+    """The 11 metric arrays of a box as contiguous FP64 arrays, each checked
+    against the padded length the C-ABI copies (NX = npx+2 for the x
+    metrics, NY = npy+2 for y, NZ = npz+2 for z, RKAPA and DKAPA); raises
+    ValueError before any library call when one is short or long."""
+    nx, ny, nz = desc.npx + 2, desc.npy + 2, desc.npz + 2
+    want = dict(zip(METRICS, (nx, nx, nx, ny, ny, ny, nz, nz, nz, nz, nz)))
+    out = []
+    for k in METRICS:
+        a = np.ascontiguousarray(meta[k], dtype=np.float64)
+        n = want[k]
+        if a.ndim != 1 or a.size != n:
+            raise ValueError(f"metric {k}: {a.shape}, expected ({n},)")
+        out.append(a)
+    return out
+
+
 class Box:
     """One resident box on the current GPU, driven through the C-ABI.
 
@@ -172,12 +189,12 @@ class Box:
         self._have_bg = False
         self.shape = (self.desc.npx + 2, self.desc.npy + 2, self.desc.npz + 2)
         self.lmag = bool(self.desc.lmag)
+        self._metrics = metric_arrays(meta, self.desc)
         h = C.c_void_p()
         rc = self.lib.mhd_create(C.byref(self.desc), C.byref(h))
         if rc:
             raise MhdError(rc, self.lib.mhd_last_error(None).decode())
         self.h = h
-        self._metrics = [np.ascontiguousarray(meta[k], dtype=np.float64) for k in METRICS]
         self._check(self.lib.mhd_set_metrics(self.h, *[_ptr(a) for a in self._metrics]))
 
     def _check(self, rc):
@@ -329,14 +346,14 @@ class Ensemble:
         self.shapes = [(d.npx + 2, d.npy + 2, d.npz + 2) for d in self.descs]
         self.lmag = bool(self.descs[0].lmag)
         self.n = len(metas)
+        metrics = [metric_arrays(m, d) for m, d in zip(metas, self.descs)]
         h = C.c_void_p()
         rc = self.lib.mhd_ens_create(self.descs, self.n, C.byref(h))
         if rc:
             raise MhdError(rc, self.lib.mhd_ens_last_error(None).decode())
         self.h = h
         self._have_bg = [False] * self.n
-        for b, m in enumerate(metas):
-            arrs = [np.ascontiguousarray(m[k], dtype=np.float64) for k in METRICS]
+        for b, arrs in enumerate(metrics):
             self._check(self.lib.mhd_ens_set_metrics(self.h, b, *[_ptr(a) for a in arrs]))
 
     def _check(self, rc):

@@ -82,3 +82,20 @@ def test_nonfinite_box_in_an_ensemble_is_reported():
         with pytest.raises(pkg.MhdError) as e:
             ens.steps(1, graph=False)
         assert e.value.code == pkg.MHD_ERR_NONFINITE
+
+
+@pytest.mark.parametrize("key,delta", [("dxxdx", -1), ("ddy", 1), ("rkapa", -2), ("ddz", -1)])
+def test_metric_lengths_checked_before_create(key, delta):
+    """A metric array of the wrong length is refused in Python before any
+    library call: the C side copies NX, NY or NZ doubles from each array."""
+    pkg = H.pkg()
+    meta, _ = random_box(16, 8, 6, True, True, seed=1)
+    a = np.asarray(meta[key])
+    meta[key] = np.resize(a, a.size + delta)
+    desc = pkg.make_desc(meta)
+    with pytest.raises(ValueError, match=key):
+        pkg.metric_arrays(meta, desc)
+    with pytest.raises(ValueError, match=key):
+        pkg.Box(meta)  # raises before mhd_create, so no device is needed
+    with pytest.raises(ValueError, match=key):
+        pkg.Ensemble([meta])
