@@ -76,14 +76,15 @@ CONFIGS = {
         steps=4, dumps=[1, 4], ampu="1.0D-02", ampb="3.0D-02", seed=7,
     ),
     # The CPU-runnable 64x64x32 box (MHD + vertical rotation, stretched z,
-    # contrast 10 polytrope, aspect 4).  Too large to commit in full; the
-    # fixture keeps checksums and sample planes.
+    # contrast 10 polytrope, aspect 4), run for the 200 steps of that
+    # workload.  Too large to commit in full; the fixture keeps checksums of
+    # every dumped step and sample planes of the last.
     "mhdrot64": dict(
         params=dict(NPX=64, NPY=64, NPZ=32, NPEY=4, NPEZ=2, LMAG=".TRUE.", LROT=".TRUE.",
                     IZCON=1, ITCON=0),
         par={1: "3.0E02", 2: "1.0E00", 3: "9.0E00", 4: "1.8E01", 5: "1.0E00",
              7: "3.0E02", 8: "1.0E02", 20: "1.0E-04", 21: "1.0E-02", 22: "0.3E00",
              23: "0.5E00"},
-        steps=3, dumps=[1, 3], ampu="1.0D-03", ampb="1.0D-03", seed=5,
+        steps=200, dumps=[1, 3, 50, 100, 200], ampu="1.0D-03", ampb="1.0D-03", seed=5,
     ),
 }

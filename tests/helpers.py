@@ -65,14 +65,15 @@ def harness_hash(ig, jg, kg, ic, seed):
     return h.astype(np.float64) / float(p) - 0.5
 
 
-def harness_initial_state(g: dict, ampu: float, ampb: float, seed: int) -> dict:
-    """Rebuild the harness's initial state from the committed 1-D profiles:
-    STATIC slicing + B layer + RAN2 noise, the harness perturbation and the
-    first COMMUNICATE."""
+def harness_initial_state(g: dict, ampu: float, ampb: float, seed: int, cols=None) -> dict:
+    """Rebuild the harness's initial state from 1-D profiles (the committed
+    ones, or cols = (T, rho, Bx) padded columns): STATIC slicing + B layer +
+    RAN2 noise, the harness perturbation and the first COMMUNICATE."""
     m = ics()
     npx, npy, npz = int(g["npx"]), int(g["npy"]), int(g["npz"])
     lmag = bool(int(g["lmag"]))
-    st = m.static_state(npx, npy, npz, g["prof_t"], g["prof_r"], g["prof_bx"] if lmag else None,
+    ct, cr, cb = cols if cols is not None else (g["prof_t"], g["prof_r"], g["prof_bx"])
+    st = m.static_state(npx, npy, npz, ct, cr, cb if lmag else None,
                         ampt=float(g["ampt"]), obeta=float(g["obeta"]), lmag=lmag)
     ig, jg, kg = np.meshgrid(np.arange(2, npx + 2, dtype=np.int64),
                              np.arange(2, npy + 2, dtype=np.int64),
