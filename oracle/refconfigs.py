@@ -88,3 +88,25 @@ CONFIGS = {
         steps=200, dumps=[1, 3, 50, 100, 200], ampu="1.0D-03", ampb="1.0D-03", seed=5,
     ),
 }
+
+# Runs of the reference's own main program (3dmhd.f) with its output files
+# (.dat0 per rank, .par, .lis): fresh start from STATIC, a dump after every
+# step.  The drop-in STEP and the host driver must write the same bytes.
+DRIVER_CONFIGS = {
+    # MHD + 45-degree rotation, Bx layer, T noise, depth-dependent kappa.
+    "drv12": dict(
+        params=dict(NPX=12, NPY=8, NPZ=8, NPEY=2, NPEZ=2, LMAG=".TRUE.", LROT=".TRUE.",
+                    IZCON=1, ITCON=0),
+        par={1: "1.5E02", 2: "1.0E00", 3: "2.0E00", 4: "4.0E00", 5: "2.0E00", 6: ANG45,
+             7: "1.2E02", 8: "5.0E01", 9: "0.6E00", 10: "0.1E00", 11: "3.0E00",
+             20: "1.0E-03", 21: "1.0E-01", 22: "0.3E00", 23: "0.5E00"},
+        ipar={3: 3, 4: 1}, nsteps=3,
+    ),
+    # Hydrodynamic, fixed-flux top and fixed-T bottom.
+    "drvh12": dict(
+        params=dict(NPX=12, NPY=8, NPZ=8, NPEY=2, NPEZ=2, IZCON=0, ITCON=1, ZZ2="1.0D+00"),
+        par={1: "2.0E02", 2: "5.0E-01", 3: "4.0E00", 4: "8.0E00", 17: "3.0E00", 18: "2.0E00",
+             20: "2.0E-03"},
+        ipar={3: 3, 4: 1}, nsteps=3,
+    ),
+}
