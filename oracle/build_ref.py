@@ -112,6 +112,38 @@ def build(name: str, cfg: dict | None = None, main_exe: bool = False) -> str:
     return d
 
 
+FORTRAN_DIR = os.path.join(REPO, "rast-3dmhd_amd", "fortran")
+MPI_LIBS = [f"-L{MPI_LIB}", f"-Wl,-rpath,{MPI_LIB}", "-lmpifort", "-lmpi"]
+
+
+def build_dropin(name: str, cfg: dict | None = None) -> str:
+    """The reference's own main program and set-up routines, unchanged, with
+    the GPU STEP shim in place of its STEP: compiles the reference for the
+    box (build()), makes the reference's STEP in 3dmhdsub.o a weak symbol
+    (objcopy, on the object compiled here) so that the shim's strong STEP is
+    the one linked, and links 3dmhd.o + 3dmhdsub.o + 3dmhdset.o + the shim +
+    librast3dmhd.so into oracle/_ref/<name>/3dmhd_gpu.exe.  Returns its path."""
+    cfg = cfg or refconfigs.DRIVER_CONFIGS[name]
+    d = build(name, cfg, main_exe=True)
+    _run(["objcopy", "--weaken-symbol=step_", "3dmhdsub.o", "3dmhdsub_nostep.o"], d)
+    _run([FC, "-O2", "-J", d, "-c", os.path.join(FORTRAN_DIR, "rast3dmhd_c.f90"), "-o", "rast3dmhd_c.o"], d)
+    _run([FC, "-O2", "-ffixed-form", "-I", d, f"-I{MPI_INC}", "-J", d, "-c",
+          os.path.join(FORTRAN_DIR, "step_shim.f"), "-o", "step_shim.o"], d)
+    libdir = os.path.join(REPO, "rast-3dmhd_amd")
+    rpath = "$ORIGIN/" + os.path.relpath(libdir, d)
+    exe = os.path.join(d, "3dmhd_gpu.exe")
+    _run([FC, "3dmhd.o", "3dmhdsub_nostep.o", "3dmhdset.o", "step_shim.o", "rast3dmhd_c.o",
+          f"-L{libdir}", "-lrast3dmhd", f"-Wl,-rpath,{rpath}", *MPI_LIBS, "-o", exe], d)
+    syms = _run(["nm", exe], d).stdout
+    if not any(line.split()[-1:] == ["step_"] and " T " in line for line in syms.splitlines()):
+        raise RuntimeError(f"{exe}: STEP is not the shim's strong symbol")
+    return exe
+
+
+def build_dropins() -> list:
+    return [build_dropin(n) for n in refconfigs.DRIVER_CONFIGS]
+
+
 def build_baseline() -> str:
     """Compile the reference main program for the throughput workload
     (refconfigs.BASELINE), used as the CPU baseline of bench.py.  Skips the

@@ -1,7 +1,8 @@
 """Writes build/<name>/3dmhdparam.f for the boxes the Fortran drop-in is
 built for.  The boxes mirror the small golden-fixture boxes (same physics
 switches, sizes and boundary codes) with the single-domain decomposition
-NPEY=NPEZ=1 that the GPU step requires."""
+NPEY=NPEZ=1 (one process, no MPI); lshr16 and id16 switch on branches that
+the GPU step does not have."""
 
 import os
 import sys
@@ -13,6 +14,9 @@ import mkparam  # noqa: E402
 BOXES = {
     "hydro16": dict(NPX=16, NPY=16, NPZ=12, IZCON=1, ITCON=0),
     "mhdrot16": dict(NPX=16, NPY=16, NPZ=12, LMAG=".TRUE.", LROT=".TRUE.", IZCON=1, ITCON=0),
+    # branches the GPU step does not have: the shim must stop before stepping
+    "lshr16": dict(NPX=16, NPY=16, NPZ=12, LMAG=".TRUE.", LROT=".TRUE.", IZCON=1, ITCON=0, LSHR=".TRUE."),
+    "id16": dict(NPX=16, NPY=16, NPZ=12, IZCON=1, ITCON=0, ID=1, DH="1.0E-02"),
 }
 
 

@@ -89,9 +89,9 @@ module rast3dmhd_c
 
 contains
 
-  ! Prints the library's last error for handle h and stops, as the
-  ! reference does on invalid settings (WRITE(6,*) ... ; STOP).
-  subroutine mhd_fail(h, what, rc)
+  ! Prints the library's last error for handle h (null: the last error of
+  ! a call without a handle) with the failing call's name.
+  subroutine mhd_report(h, what, rc)
     type(c_ptr), intent(in) :: h
     character(len=*), intent(in) :: what
     integer(c_int), intent(in) :: rc
@@ -109,6 +109,15 @@ contains
       end do
     end if
     write(6, '(A,A,I3,A,A)') what, ': error ', rc, ': ', trim(line)
+  end subroutine mhd_report
+
+  ! mhd_report, then stop, as the reference does on invalid settings
+  ! (WRITE(6,*) ... ; STOP) in a single-process run.
+  subroutine mhd_fail(h, what, rc)
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    integer(c_int), intent(in) :: rc
+    call mhd_report(h, what, rc)
     stop
   end subroutine mhd_fail
 end module rast3dmhd_c

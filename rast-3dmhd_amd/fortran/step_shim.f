@@ -17,14 +17,28 @@ C     (MOD(NIT,NSTEP0).EQ.0), the last step (NIT.GE.NTOTAL) and the
 C     DT.LT.1.0E-08 abort path (3dmhd.f:924-1009, 1053-1153).  Between
 C     those, the device holds the authoritative state.
 C
-C     The whole domain is resident on one GPU: the parameter block must
-C     have NPEY=NPEZ=1.  Errors print the library's message and STOP, as
-C     the reference does for invalid settings.
+C     Decomposition.  The driver runs on NPE=NPEY*NPEZ MPI ranks, as the
+C     reference requires (3dmhdparam.f:11; rank MYPE = MYPEY+NPEY*MYPEZ
+C     holds rows MYPEY*NRY+1.. and planes MYPEZ*NRZ+1.., 3dmhd.f:87-88).
+C     The whole domain is resident on the GPU of rank 0: on the first
+C     call every rank's slab of /BIG/ and of the y and z metrics is
+C     gathered there, and rank 0 alone creates the handle and steps.  DT
+C     and UMACH are broadcast every step.  Where the driver reads /BIG/,
+C     rank 0 downloads the whole domain and scatters every rank's slab
+C     back with its ghost layers as COMMUNICATE leaves them (neighbour
+C     rows and planes, periodic wrap; the untouched physical z ghosts).
+C     With NPE=1 no MPI call is made and /BIG/ is used directly.
+C
+C     Branches of FLUXES/BCON that the GPU step does not have (ID, LREM,
+C     LSHR, RLAX; ITCON outside 0/1 and TP are refused by mhd_create)
+C     stop the run before the first step.  Errors print the library's
+C     message and stop, as the reference does for invalid settings.
 C=======================================================================
       SUBROUTINE STEP
       USE ISO_C_BINDING
       USE RAST3DMHD_C
       INCLUDE '3dmhdparam.f'
+      INCLUDE 'mpif.h'
 C
       DIMENSION RU(NX,NY,NZ),RV(NX,NY,NZ),RW(NX,NY,NZ),RO(NX,NY,NZ)
      2         ,TT(NX,NY,NZ)
@@ -65,120 +79,346 @@ C
       COMMON/CTIM/DT,TIMT,TIMC,TIMI
       COMMON/TRACE/UMACH
       COMMON/RUNGKU/GAM1,GAM2,GAM3,ZETA1,ZETA2
+      COMMON/COMMUN/MYPE,MYPEY,MYPEZ,MPISIZE
+      COMMON/RELAX/TSTART,TOFF,RLAX
 C
       TYPE(C_PTR), SAVE :: H
       LOGICAL, SAVE :: READY = .FALSE.
+C     rank 0 of a decomposed run: the whole padded domain, /BIG/ order
+C     RU,RV,RW,RO,TT,BX,BY,BZ, and the y and z metric arrays
+      REAL(C_DOUBLE), ALLOCATABLE, SAVE :: G(:,:,:,:)
+      REAL(C_DOUBLE), ALLOCATABLE, SAVE :: GY(:,:), GZ(:,:)
+      REAL(C_DOUBLE), ALLOCATABLE :: BUF(:,:,:,:), MV(:), MB(:,:)
       TYPE(MHD_DESC) :: D
       INTEGER(C_INT) :: IRC
-      REAL(C_DOUBLE) :: DTC, UMC
+      REAL(C_DOUBLE) :: DTC, UMC, V3(3), TBG(1)
       CHARACTER(LEN=16) :: PREC
-      INTEGER :: PLEN, PST, IB, JB, KB
-      REAL(C_DOUBLE) :: RO0(NZ), TT0(NZ)
+      CHARACTER(LEN=40) :: WHAT
+      INTEGER :: PLEN, PST, NF, IERR, IR, PY, PZ, J, K, JG, KG
+      INTEGER :: NSEND, NMV
+      LOGICAL :: DUMP, FL
 C
       IF (.NOT.READY) THEN
-         IF (NPEY*NPEZ.NE.1) THEN
-            WRITE(6,*)'STEP: the GPU step needs NPEY=NPEZ=1'
-            STOP
-         ENDIF
          IF ((IXC.NE.0).OR.(IYC.NE.0)) THEN
             WRITE(6,*)'FLUXES: Non-periodic horizontal boundaries'
-            STOP
+            CALL SHIM_STOP
          ENDIF
-         D%NPX=NPX
-         D%NPY=NPY
-         D%NPZ=NPZ
-         D%LMAG=0
-         IF (LMAG) D%LMAG=1
-         D%LROT=0
-         IF (LROT) D%LROT=1
-         D%IZCON=IZC
-         D%ITCON=ITC
-         D%IBCON=IBC
-C        RAST3DMHD_PRECISION=mixed selects the mixed-precision form
-         D%PRECISION=MHD_FP64
-         CALL GET_ENVIRONMENT_VARIABLE('RAST3DMHD_PRECISION',PREC,
-     2                                 PLEN,PST)
-         IF ((PST.EQ.0).AND.(PLEN.GT.0)) THEN
-            IF (PREC(1:PLEN).EQ.'mixed') THEN
-               D%PRECISION=MHD_MIXED
-            ELSE IF (PREC(1:PLEN).NE.'fp64') THEN
-               WRITE(6,*)'STEP: RAST3DMHD_PRECISION is fp64 or mixed'
-               STOP
+         IF ((ID.NE.0).OR.LREM.OR.LSHR.OR.(RLAX.GT.0.0E00)) THEN
+            WRITE(6,*)'STEP: the GPU step has no ID, LREM, LSHR or ',
+     2                'RLAX branch'
+            CALL SHIM_STOP
+         ENDIF
+         NF=5
+         IF (LMAG) NF=8
+         NSEND=NX*NY*NZ
+         TBG(1)=TB
+         IF (NPE.GT.1) THEN
+C           TB is the bottom ranks' value on a restart (3dmhd.f:354-356)
+            CALL MPI_BCAST(TBG,1,MPI_DOUBLE_PRECISION,NPE-1,
+     2                     MPI_COMM_WORLD,IERR)
+            IF (MYPE.EQ.0) THEN
+               ALLOCATE(G(NX,NPY+2,NPZ+2,NF),BUF(NX,NY,NZ,0:NPE-1))
+               ALLOCATE(GY(NPY+2,3),GZ(NPZ+2,5))
+            ELSE
+               ALLOCATE(BUF(1,1,1,0:0))
             ENDIF
+            CALL GATHER3(RU,1)
+            CALL GATHER3(RV,2)
+            CALL GATHER3(RW,3)
+            CALL GATHER3(RO,4)
+            CALL GATHER3(TT,5)
+            IF (LMAG) THEN
+               CALL GATHER3(BX,6)
+               CALL GATHER3(BY,7)
+               CALL GATHER3(BZ,8)
+            ENDIF
+            CALL GATHER_METRICS
          ENDIF
-         D%HX=HX
-         D%H2X=H2X
-         D%HY=HY
-         D%H2Y=H2Y
-         D%HZ=HZ
-         D%H2Z=H2Z
-         D%C13=C13
-         D%C23=C23
-         D%C43=C43
-         D%GAM1=GAM1
-         D%GAM2=GAM2
-         D%GAM3=GAM3
-         D%ZETA1=ZETA1
-         D%ZETA2=ZETA2
-         D%CV=CV
-         D%OCV=OCV
-         D%ORE=ORE
-         D%RE=RE
-         D%REPR=REPR
-         D%THETA=THETA
-         D%GRAV=GRAV
-         D%SF=SF
-         D%GAMMA=GAMMA
-         D%OMX=OMX
-         D%OMZ=OMZ
-         D%ORM=ORM
-         D%RM=RM
-         D%OBETA=OBETA
-         D%TB=TB
-         D%TP=TP
-         D%DZF=1.0E00/FLOAT(NPZ-1)
-         IRC=MHD_CREATE(D,H)
-         IF (IRC.NE.0) CALL MHD_FAIL(C_NULL_PTR,'STEP: mhd_create',IRC)
-         IRC=MHD_SET_METRICS(H,DXXDX,D2XXDX2,DDX,DYYDY,D2YYDY2,DDY,
-     2                       DZZDZ,D2ZZDZ2,DDZ,RKAPA,DKAPA)
-         IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_set_metrics',IRC)
-         IF (D%PRECISION.EQ.MHD_MIXED) THEN
-C           background of the mixed form: horizontal means of rho and T
-            DO KB=1,NZ
-               RO0(KB)=0.0D0
-               TT0(KB)=0.0D0
-               DO JB=2,NY-1
-                  DO IB=2,NX-1
-                     RO0(KB)=RO0(KB)+RO(IB,JB,KB)
-                     TT0(KB)=TT0(KB)+TT(IB,JB,KB)
-                  ENDDO
-               ENDDO
-               RO0(KB)=RO0(KB)/DBLE((NX-2)*(NY-2))
-               TT0(KB)=TT0(KB)/DBLE((NX-2)*(NY-2))
-            ENDDO
-            IRC=MHD_SET_BACKGROUND(H,RO0,TT0)
-            IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_set_background',
-     2                                  IRC)
+         IRC=0
+         IF (MYPE.EQ.0) CALL SHIM_CREATE
+         V3(3)=DBLE(IRC)
+         IF (NPE.GT.1) CALL MPI_BCAST(V3(3:3),1,MPI_DOUBLE_PRECISION,
+     2                                0,MPI_COMM_WORLD,IERR)
+         IF (V3(3).NE.0.0D0) THEN
+            IF (MYPE.EQ.0) CALL MHD_REPORT(H,TRIM(WHAT),
+     2                                     INT(V3(3),C_INT))
+            CALL SHIM_STOP
          ENDIF
-         IRC=MHD_UPLOAD(H,RU,RV,RW,RO,TT,BX,BY,BZ)
-         IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_upload',IRC)
-         IRC=MHD_SET_CLOCK(H,INT(NIT,C_INT),TIMC,TIMI)
-         IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_set_clock',IRC)
          READY=.TRUE.
       ENDIF
 C
-      IRC=MHD_STEP(H,DTC,UMC)
-      IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_step',IRC)
-      DT=DTC
-      UMACH=UMC
+      IF (MYPE.EQ.0) THEN
+         IRC=MHD_STEP(H,DTC,UMC)
+         V3(1)=DTC
+         V3(2)=UMC
+         V3(3)=DBLE(IRC)
+      ENDIF
+      IF (NPE.GT.1) CALL MPI_BCAST(V3,3,MPI_DOUBLE_PRECISION,0,
+     2                             MPI_COMM_WORLD,IERR)
+      IF (V3(3).NE.0.0D0) THEN
+         IF (MYPE.EQ.0) CALL MHD_REPORT(H,'STEP: mhd_step',
+     2                                  INT(V3(3),C_INT))
+         CALL SHIM_STOP
+      ENDIF
+      DT=V3(1)
+      UMACH=V3(2)
       NIT=NIT+1
       TIMC=TIMC+DT
       TIMT=TIMI+TIMC
 C
-      IF ((MOD(NIT,NSTEP0).EQ.0).OR.(NIT.GE.NTOTAL)
-     2    .OR.(DT.LT.1.0E-08)) THEN
-         IRC=MHD_DOWNLOAD(H,RU,RV,RW,RO,TT,BX,BY,BZ)
-         IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_download',IRC)
+      DUMP=(MOD(NIT,NSTEP0).EQ.0).OR.(NIT.GE.NTOTAL)
+     2     .OR.(DT.LT.1.0E-08)
+      IF (DUMP) THEN
+         IF (NPE.EQ.1) THEN
+            IRC=MHD_DOWNLOAD(H,RU,RV,RW,RO,TT,BX,BY,BZ)
+            IF (IRC.NE.0) CALL MHD_FAIL(H,'STEP: mhd_download',IRC)
+         ELSE
+            IF (MYPE.EQ.0) THEN
+               IF (LMAG) THEN
+                  IRC=MHD_DOWNLOAD(H,G(:,:,:,1),G(:,:,:,2),G(:,:,:,3),
+     2                 G(:,:,:,4),G(:,:,:,5),G(:,:,:,6),G(:,:,:,7),
+     3                 G(:,:,:,8))
+               ELSE
+C                 (BX,BY,BZ of /BIG/ are not touched without LMAG)
+                  IRC=MHD_DOWNLOAD(H,G(:,:,:,1),G(:,:,:,2),G(:,:,:,3),
+     2                 G(:,:,:,4),G(:,:,:,5),BX,BY,BZ)
+               ENDIF
+               V3(3)=DBLE(IRC)
+            ENDIF
+            CALL MPI_BCAST(V3(3:3),1,MPI_DOUBLE_PRECISION,0,
+     2                     MPI_COMM_WORLD,IERR)
+            IF (V3(3).NE.0.0D0) THEN
+               IF (MYPE.EQ.0) CALL MHD_REPORT(H,'STEP: mhd_download',
+     2                                        INT(V3(3),C_INT))
+               CALL SHIM_STOP
+            ENDIF
+            ALLOCATE(BUF(NX,NY,NZ,0:MERGE(NPE-1,0,MYPE.EQ.0)))
+            CALL SCATTER3(RU,1)
+            CALL SCATTER3(RV,2)
+            CALL SCATTER3(RW,3)
+            CALL SCATTER3(RO,4)
+            CALL SCATTER3(TT,5)
+            IF (LMAG) THEN
+               CALL SCATTER3(BX,6)
+               CALL SCATTER3(BY,7)
+               CALL SCATTER3(BZ,8)
+            ENDIF
+            DEALLOCATE(BUF)
+         ENDIF
       ENDIF
       RETURN
+C
+      CONTAINS
+C
+C     The reference's stop on invalid settings: print, finalise MPI
+C     (every rank reaches this together), stop.
+      SUBROUTINE SHIM_STOP
+      INTEGER :: IE
+      CALL MPI_FINALIZED(FL,IE)
+      IF (.NOT.FL) THEN
+         CALL MPI_INITIALIZED(FL,IE)
+         IF (FL) CALL MPI_FINALIZE(IE)
+      ENDIF
+      STOP
+      END SUBROUTINE SHIM_STOP
+C
+C     Rank 0: local array LOC of every rank into G(:,:,:,F); interior
+C     rows and planes from their owner, the outer ghost rows and planes
+C     from the edge ranks.
+      SUBROUTINE GATHER3(LOC,F)
+      REAL*8 :: LOC(NX,NY,NZ)
+      INTEGER :: F
+      CALL MPI_GATHER(LOC,NSEND,MPI_DOUBLE_PRECISION,BUF,NSEND,
+     2                MPI_DOUBLE_PRECISION,0,MPI_COMM_WORLD,IERR)
+      IF (MYPE.NE.0) RETURN
+      DO IR=0,NPE-1
+         PY=MOD(IR,NPEY)
+         PZ=IR/NPEY
+         DO K=1,NZ
+            IF ((K.EQ.1).AND.(PZ.GT.0)) CYCLE
+            IF ((K.EQ.NZ).AND.(PZ.LT.NPEZ-1)) CYCLE
+            KG=PZ*NRZ+K
+            DO J=1,NY
+               IF ((J.EQ.1).AND.(PY.GT.0)) CYCLE
+               IF ((J.EQ.NY).AND.(PY.LT.NPEY-1)) CYCLE
+               JG=PY*NRY+J
+               G(:,JG,KG,F)=BUF(:,J,K,IR)
+            ENDDO
+         ENDDO
+      ENDDO
+      END SUBROUTINE GATHER3
+C
+C     Rank 0 to every rank: its window of G(:,:,:,F), ghosts included.
+      SUBROUTINE SCATTER3(LOC,F)
+      REAL*8 :: LOC(NX,NY,NZ)
+      INTEGER :: F
+      IF (MYPE.EQ.0) THEN
+         DO IR=0,NPE-1
+            PY=MOD(IR,NPEY)
+            PZ=IR/NPEY
+            BUF(:,:,:,IR)=G(:,PY*NRY+1:PY*NRY+NY,PZ*NRZ+1:PZ*NRZ+NZ,F)
+         ENDDO
+      ENDIF
+      CALL MPI_SCATTER(BUF,NSEND,MPI_DOUBLE_PRECISION,LOC,NSEND,
+     2                 MPI_DOUBLE_PRECISION,0,MPI_COMM_WORLD,IERR)
+      END SUBROUTINE SCATTER3
+C
+C     Rank 0: the y metrics (DYYDY, D2YYDY2, DDY) and z metrics (DZZDZ,
+C     D2ZZDZ2, DDZ, RKAPA, DKAPA) of the whole domain from the slabs.
+      SUBROUTINE GATHER_METRICS
+      NMV=3*NY+5*NZ
+      ALLOCATE(MV(NMV))
+      IF (MYPE.EQ.0) THEN
+         ALLOCATE(MB(NMV,0:NPE-1))
+      ELSE
+         ALLOCATE(MB(1,0:0))
+      ENDIF
+      MV(1:NY)=DYYDY
+      MV(NY+1:2*NY)=D2YYDY2
+      MV(2*NY+1:3*NY)=DDY
+      MV(3*NY+1:3*NY+NZ)=DZZDZ
+      MV(3*NY+NZ+1:3*NY+2*NZ)=D2ZZDZ2
+      MV(3*NY+2*NZ+1:3*NY+3*NZ)=DDZ
+      MV(3*NY+3*NZ+1:3*NY+4*NZ)=RKAPA
+      MV(3*NY+4*NZ+1:3*NY+5*NZ)=DKAPA
+      CALL MPI_GATHER(MV,NMV,MPI_DOUBLE_PRECISION,MB,NMV,
+     2                MPI_DOUBLE_PRECISION,0,MPI_COMM_WORLD,IERR)
+      IF (MYPE.EQ.0) THEN
+         DO IR=0,NPE-1
+            PY=MOD(IR,NPEY)
+            PZ=IR/NPEY
+            DO J=1,NY
+               IF ((J.EQ.1).AND.(PY.GT.0)) CYCLE
+               IF ((J.EQ.NY).AND.(PY.LT.NPEY-1)) CYCLE
+               JG=PY*NRY+J
+               GY(JG,1)=MB(J,IR)
+               GY(JG,2)=MB(NY+J,IR)
+               GY(JG,3)=MB(2*NY+J,IR)
+            ENDDO
+            DO K=1,NZ
+               IF ((K.EQ.1).AND.(PZ.GT.0)) CYCLE
+               IF ((K.EQ.NZ).AND.(PZ.LT.NPEZ-1)) CYCLE
+               KG=PZ*NRZ+K
+               GZ(KG,1)=MB(3*NY+K,IR)
+               GZ(KG,2)=MB(3*NY+NZ+K,IR)
+               GZ(KG,3)=MB(3*NY+2*NZ+K,IR)
+               GZ(KG,4)=MB(3*NY+3*NZ+K,IR)
+               GZ(KG,5)=MB(3*NY+4*NZ+K,IR)
+            ENDDO
+         ENDDO
+      ENDIF
+      DEALLOCATE(MV,MB)
+      END SUBROUTINE GATHER_METRICS
+C
+C     Rank 0: handle, metrics, background (mixed form), state, clock.
+C     A failure leaves its code in IRC and the failing call in WHAT.
+      SUBROUTINE SHIM_CREATE
+      REAL(C_DOUBLE) :: RO0(NPZ+2), TT0(NPZ+2)
+      INTEGER :: IB, JB, KB
+      D%NPX=NPX
+      D%NPY=NPY
+      D%NPZ=NPZ
+      D%LMAG=0
+      IF (LMAG) D%LMAG=1
+      D%LROT=0
+      IF (LROT) D%LROT=1
+      D%IZCON=IZC
+      D%ITCON=ITC
+      D%IBCON=IBC
+C     RAST3DMHD_PRECISION=mixed selects the mixed-precision form
+      D%PRECISION=MHD_FP64
+      CALL GET_ENVIRONMENT_VARIABLE('RAST3DMHD_PRECISION',PREC,
+     2                              PLEN,PST)
+      IF ((PST.EQ.0).AND.(PLEN.GT.0)) THEN
+         IF (PREC(1:PLEN).EQ.'mixed') THEN
+            D%PRECISION=MHD_MIXED
+         ELSE IF (PREC(1:PLEN).NE.'fp64') THEN
+            WRITE(6,*)'STEP: RAST3DMHD_PRECISION is fp64 or mixed'
+            IRC=1
+            WHAT='STEP: RAST3DMHD_PRECISION'
+            RETURN
+         ENDIF
+      ENDIF
+      D%HX=HX
+      D%H2X=H2X
+      D%HY=HY
+      D%H2Y=H2Y
+      D%HZ=HZ
+      D%H2Z=H2Z
+      D%C13=C13
+      D%C23=C23
+      D%C43=C43
+      D%GAM1=GAM1
+      D%GAM2=GAM2
+      D%GAM3=GAM3
+      D%ZETA1=ZETA1
+      D%ZETA2=ZETA2
+      D%CV=CV
+      D%OCV=OCV
+      D%ORE=ORE
+      D%RE=RE
+      D%REPR=REPR
+      D%THETA=THETA
+      D%GRAV=GRAV
+      D%SF=SF
+      D%GAMMA=GAMMA
+      D%OMX=OMX
+      D%OMZ=OMZ
+      D%ORM=ORM
+      D%RM=RM
+      D%OBETA=OBETA
+      D%TB=TBG(1)
+      D%TP=TP
+      D%DZF=1.0E00/FLOAT(NPZ-1)
+      WHAT='STEP: mhd_create'
+      IRC=MHD_CREATE(D,H)
+      IF (IRC.NE.0) GOTO 9
+      WHAT='STEP: mhd_set_metrics'
+      IF (NPE.EQ.1) THEN
+         IRC=MHD_SET_METRICS(H,DXXDX,D2XXDX2,DDX,DYYDY,D2YYDY2,DDY,
+     2                       DZZDZ,D2ZZDZ2,DDZ,RKAPA,DKAPA)
+      ELSE
+         IRC=MHD_SET_METRICS(H,DXXDX,D2XXDX2,DDX,GY(:,1),GY(:,2),
+     2                       GY(:,3),GZ(:,1),GZ(:,2),GZ(:,3),GZ(:,4),
+     3                       GZ(:,5))
+      ENDIF
+      IF (IRC.NE.0) GOTO 9
+      IF (D%PRECISION.EQ.MHD_MIXED) THEN
+C        background of the mixed form: horizontal means of rho and T
+         DO KB=1,NPZ+2
+            RO0(KB)=0.0D0
+            TT0(KB)=0.0D0
+            DO JB=2,NPY+1
+               DO IB=2,NX-1
+                  IF (NPE.EQ.1) THEN
+                     RO0(KB)=RO0(KB)+RO(IB,JB,KB)
+                     TT0(KB)=TT0(KB)+TT(IB,JB,KB)
+                  ELSE
+                     RO0(KB)=RO0(KB)+G(IB,JB,KB,4)
+                     TT0(KB)=TT0(KB)+G(IB,JB,KB,5)
+                  ENDIF
+               ENDDO
+            ENDDO
+            RO0(KB)=RO0(KB)/DBLE(NPX*NPY)
+            TT0(KB)=TT0(KB)/DBLE(NPX*NPY)
+         ENDDO
+         WHAT='STEP: mhd_set_background'
+         IRC=MHD_SET_BACKGROUND(H,RO0,TT0)
+         IF (IRC.NE.0) GOTO 9
+      ENDIF
+      WHAT='STEP: mhd_upload'
+      IF (NPE.EQ.1) THEN
+         IRC=MHD_UPLOAD(H,RU,RV,RW,RO,TT,BX,BY,BZ)
+      ELSE IF (LMAG) THEN
+         IRC=MHD_UPLOAD(H,G(:,:,:,1),G(:,:,:,2),G(:,:,:,3),G(:,:,:,4),
+     2                  G(:,:,:,5),G(:,:,:,6),G(:,:,:,7),G(:,:,:,8))
+      ELSE
+         IRC=MHD_UPLOAD(H,G(:,:,:,1),G(:,:,:,2),G(:,:,:,3),G(:,:,:,4),
+     2                  G(:,:,:,5),BX,BY,BZ)
+      ENDIF
+      IF (IRC.NE.0) GOTO 9
+      WHAT='STEP: mhd_set_clock'
+      IRC=MHD_SET_CLOCK(H,INT(NIT,C_INT),TIMC,TIMI)
+    9 CONTINUE
+      END SUBROUTINE SHIM_CREATE
       END

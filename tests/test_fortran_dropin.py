@@ -118,3 +118,18 @@ def test_shim_rejects_unknown_precision(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
     assert "RAST3DMHD_PRECISION is fp64 or mixed" in r.stdout + r.stderr
     assert not (tmp_path / "replay.out").exists()
+
+
+@pytest.mark.parametrize("box,data", [("lshr16", "mhdrot16"), ("id16", "hydro16")])
+def test_fortran_dropin_refuses_missing_branches(tmp_path, box, data):
+    """A parameter block with a FLUXES/BCON branch the GPU step does not
+    have (LSHR, ID; likewise LREM and RLAX > 0) stops before the first step
+    with a message, as the reference stops on invalid settings: no device
+    call is made, so this runs without a GPU."""
+    exe = os.path.join(FBUILD, box, "rastreplay")
+    assert os.path.exists(exe), "build the Fortran drop-in first (__graft_entry__.build())"
+    g = H.load(data)
+    write_replay_input(tmp_path / "replay.in", g, H.initial_state(data), 1)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert "the GPU step has no ID, LREM, LSHR or RLAX branch" in r.stdout, r.stdout + r.stderr
+    assert not os.path.exists(tmp_path / "replay.out")

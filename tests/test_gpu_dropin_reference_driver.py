@@ -1,0 +1,101 @@
+"""The reference's own, unchanged main program with the GPU STEP.
+
+oracle/build_ref.build_dropin links the reference's 3dmhd.f, 3dmhdsub.f
+(its STEP made a weak symbol) and the generated 3dmhdset.f with the STEP
+shim (rast-3dmhd_amd/fortran/step_shim.f) and librast3dmhd.so, for the
+decks of tests/golden/drv12 and drvh12 (NPEY = NPEZ = 2, the decomposition
+the reference's set-up supports).  Run under MPI on 4 ranks, the shim
+gathers the slabs onto rank 0, which steps on the GPU; DT and UMACH are
+broadcast and the slabs scattered back on dump steps.  The files it writes
+must be the reference CPU build's own files byte for byte: every rank's
+rast3d.dat0.NNNN, rast3d.par and the rast3d.lis log (fresh run), and the
+restart from the reference's dump 1 continues as the uninterrupted run.
+
+The executables are built where the reference sources exist; the tests
+skip with that reason elsewhere."""
+
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import helpers as H
+from test_driver_files import GOLD, NAMES, deck, dio, layout, same_files
+
+REF = os.path.join(H.REPO, "oracle", "_ref")
+MPIEXEC = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
+
+
+def exe_of(name):
+    exe = os.path.join(REF, name, "3dmhd_gpu.exe")
+    if not os.path.exists(exe):
+        pytest.skip(f"{exe} not built (needs the reference sources: oracle/build_ref.build_dropins)")
+    return exe
+
+
+def run(name, cwd, ntotal, nstart=0, precision=None, timeout=120):
+    env = dict(os.environ, RAST_NTOTAL=str(ntotal), RAST_NSTEP0="1", RAST_NSTART=str(nstart))
+    if precision:
+        env["RAST3DMHD_PRECISION"] = precision
+    _, c = deck(name)
+    npe = int(c["params"]["NPEY"]) * int(c["params"]["NPEZ"])
+    return subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(npe), exe_of(name)], cwd=cwd, env=env,
+                          capture_output=True, text=True, timeout=timeout)
+
+
+def test_dropin_driver_stops_every_rank_on_a_shim_error(tmp_path):
+    """An invalid setting found on rank 0 stops all ranks together (no
+    rank is left waiting in a collective): checked before any device call,
+    so this runs without a GPU."""
+    r = run("drv12", tmp_path, 3, precision="bogus", timeout=60)
+    assert "RAST3DMHD_PRECISION is fp64 or mixed" in r.stdout, r.stdout + r.stderr
+    assert not os.path.exists(tmp_path / "rast3d.dat0.0000")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", NAMES)
+def test_reference_driver_with_gpu_step_writes_the_reference_files(tmp_path, name):
+    print("case reference driver + GPU STEP, fresh run", name, flush=True)
+    _, c = deck(name)
+    r = run(name, tmp_path, int(c["nsteps"]))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert same_files(os.path.join(GOLD, name), tmp_path) == []
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", NAMES)
+def test_reference_driver_with_gpu_step_restarts_from_the_reference_dump(tmp_path, name):
+    print("case reference driver + GPU STEP, restart", name, flush=True)
+    _, c = deck(name)
+    for f in os.listdir(os.path.join(GOLD, name)):
+        if not f.endswith(".lis"):
+            shutil.copy(os.path.join(GOLD, name, f), tmp_path)
+    r = run(name, tmp_path, int(c["nsteps"]) - 1, nstart=1)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lay, lmag = layout(c), name == "drv12"
+    for k in (1, 2):
+        a = dio().read_dump(os.path.join(tmp_path, "rast3d"), k, lay, lmag)
+        b = dio().read_dump(os.path.join(GOLD, name, "rast3d"), k + 1, lay, lmag)
+        for f in a:
+            assert np.array_equal(a[f], b[f]), (k, f)
+    if name == "drvh12":
+        assert same_files(os.path.join(GOLD, name + "_restart"), tmp_path) == []
+
+
+@pytest.mark.gpu
+def test_reference_driver_with_mixed_gpu_step(tmp_path):
+    print("case reference driver + GPU STEP, mixed form", flush=True)
+    _, c = deck("drv12")
+    r = run("drv12", tmp_path, int(c["nsteps"]), precision="mixed")
+    assert r.returncode == 0, r.stdout + r.stderr
+    lay = layout(c)
+    for k in (1, 2, 3):
+        a = dio().read_dump(os.path.join(tmp_path, "rast3d"), k, lay, True)
+        b = dio().read_dump(os.path.join(GOLD, "drv12", "rast3d"), k, lay, True)
+        bscale = np.sqrt(sum(np.mean(b[f] ** 2) for f in ("bx", "by", "bz")))
+        for f in a:
+            scale = bscale if f in ("bx", "by", "bz") else np.sqrt(np.mean(b[f] ** 2))
+            e = np.sqrt(np.mean((a[f] - b[f]) ** 2)) / scale
+            assert e <= 1e-6, (k, f, e)
