@@ -160,6 +160,41 @@ int mhd_flux_profiles(mhd_t *h, double *ttm, double *rom, double *fconm, double 
 int mhd_time_steps(mhd_t *h, int n, double *ms_step, double *ms_stage, double *ms_cfl,
                    double *ms_bcon);
 
+/* Device pointers to the 11 metric arrays of mhd_set_metrics (same padded
+ * lengths NX, NY, NZ), for mhd_step_noalloc. */
+typedef struct mhd_dev_metrics {
+    const double *dxxdx, *d2xxdx2, *ddx;
+    const double *dyydy, *d2yydy2, *ddy;
+    const double *dzzdz, *d2zzdz2, *ddz;
+    const double *rkapa, *dkapa;
+} mhd_dev_metrics;
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): one plain FP64 step, exactly as mhd_upload + mhd_step +
+ * mhd_download on a handle, on arrays the caller keeps on the device.  No
+ * handle, and no hipMalloc, hipFree or other allocation in the call.
+ *  - desc: as for mhd_create; precision must be MHD_FP64.
+ *  - met: device metric arrays.  state[0..7]: device pointers to the padded
+ *    Fortran-order RU,RV,RW,RO,TT,BX,BY,BZ (state[5..7] may be NULL when
+ *    LMAG is off), read at entry and overwritten with the stepped state,
+ *    ghosts filled as mhd_download leaves them.
+ *  - nit, timc, timt: /ITER/ and /CTIM/ clock in and out (NIT=NIT+1,
+ *    TIMC=TIMC+DT, TIMT=TIMI+TIMC); timi is TIMI.  dt, umach: the DT used
+ *    and UMACH of the state at entry (either may be NULL).
+ *  - stream: a hipStream_t, or NULL for the null stream.  The call returns
+ *    after the stream has finished the step.
+ * The two RK registers, the Z register and the reduction scalars live in
+ * one static device array of at most 64 MiB per GPU (held until the process
+ * ends), so 3 * nf * 8 * NPX*NPY*(NPZ+2) bytes (nf = 8 with LMAG, else 5)
+ * must fit in 64 MiB - 256 B (e.g. 64x64x32 MHD); larger boxes are rejected
+ * with MHD_ERR_ARG.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other threads or streams) race.  Errors are reported
+ * through mhd_last_error(NULL). */
+int mhd_step_noalloc(const mhd_desc *desc, const mhd_dev_metrics *met, double *const *state,
+                     int *nit, double *timc, double *timt, double timi,
+                     double *dt, double *umach, void *stream);
+
 /* Human-readable description of the last error on this handle (or of the
  * last failed mhd_create when h is NULL). */
 const char *mhd_last_error(const mhd_t *h);

@@ -37,6 +37,11 @@ class MhdDesc(C.Structure):
     _fields_ = [(k, C.c_int) for k in _DESC_INTS] + [(k, C.c_double) for k in _DESC_DBLS]
 
 
+class MhdDevMetrics(C.Structure):
+    """mhd_dev_metrics: device pointers to the 11 metric arrays."""
+    _fields_ = [(k, C.c_void_p) for k in METRICS]
+
+
 class MhdError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -71,6 +76,9 @@ def load(path: str | None = None):
     lib.mhd_box_filter.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _P]
     lib.mhd_time_steps.argtypes = [C.c_void_p, C.c_int, _P, _P, _P, _P]
     lib.mhd_flux_profiles.argtypes = [C.c_void_p, _P, _P, _P, _P]
+    lib.mhd_step_noalloc.argtypes = [C.POINTER(MhdDesc), C.POINTER(MhdDevMetrics), C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_int), _P, _P, C.c_double, _P, _P, C.c_void_p]
+    lib.mhd_step_noalloc.restype = C.c_int
     lib.mhd_last_error.argtypes = [C.c_void_p]
     lib.mhd_last_error.restype = C.c_char_p
     lib.mhd_destroy.argtypes = [C.c_void_p]

@@ -71,6 +71,14 @@ static int fail(mhd_t *h, int code, const char *fmt, ...) {
     return code;
 }
 
+int rmhd::global_fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_create_err, sizeof g_create_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
 #define HIPCHK(h, x)                                                                       \
     do {                                                                                   \
         hipError_t e_ = (x);                                                               \

@@ -11,6 +11,10 @@
 
 namespace rmhd {
 
+// Records an error for mhd_last_error(NULL) (the message of calls without a
+// handle) and returns code (mhd_capi.hip).
+int global_fail(int code, const char *fmt, ...);
+
 // Checks a descriptor; on failure writes a message to msg and returns the
 // MHD_ERR_* code.
 inline int check_desc(const mhd_desc *d, char *msg, size_t len) {

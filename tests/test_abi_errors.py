@@ -99,3 +99,35 @@ def test_metric_lengths_checked_before_create(key, delta):
         pkg.Box(meta)  # raises before mhd_create, so no device is needed
     with pytest.raises(ValueError, match=key):
         pkg.Ensemble([meta])
+
+
+def _noalloc_call(meta, **over):
+    pkg = H.pkg()
+    lib = pkg.load()
+    d = pkg.make_desc(meta)
+    for k, v in over.items():
+        setattr(d, k, v)
+    dm = pkg.MhdDevMetrics()
+    ptrs = (C.c_void_p * 8)()
+    nit, timc, timt = C.c_int(0), C.c_double(0.0), C.c_double(0.0)
+    rc = lib.mhd_step_noalloc(C.byref(d), C.byref(dm), ptrs, C.byref(nit), C.byref(timc), C.byref(timt), 0.0,
+                              None, None, None)
+    return rc, lib.mhd_last_error(None).decode()
+
+
+def test_noalloc_step_rejects_before_touching_the_device():
+    """mhd_step_noalloc checks its arguments, the precision and the size of
+    its static scratch before any HIP call."""
+    pkg = H.pkg()
+    meta, _ = random_box(128, 128, 64, True, True, seed=1)  # 3 registers x 8 fields: 207 MB > 64 MiB
+    rc, msg = _noalloc_call(meta)
+    assert rc == pkg.MHD_ERR_ARG and "scratch" in msg and "maximum" in msg, (rc, msg)
+    meta, _ = random_box(16, 8, 6, True, True, seed=1)
+    rc, msg = _noalloc_call(meta, precision=pkg.MHD_MIXED)
+    assert rc == pkg.MHD_ERR_SCOPE and "MHD_FP64" in msg, (rc, msg)
+    rc, msg = _noalloc_call(meta, itcon=3)
+    assert rc == pkg.MHD_ERR_SCOPE and "ITCON=3" in msg, (rc, msg)
+    rc, msg = _noalloc_call(meta)  # fits, but the metric pointers are null
+    assert rc == pkg.MHD_ERR_ARG and "metric array 0 is null" in msg, (rc, msg)
+    lib = pkg.load()
+    assert lib.mhd_step_noalloc(None, None, None, None, None, None, 0.0, None, None, None) == pkg.MHD_ERR_ARG
