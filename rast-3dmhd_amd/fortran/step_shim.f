@@ -98,6 +98,7 @@ C     RU,RV,RW,RO,TT,BX,BY,BZ, and the y and z metric arrays
       INTEGER :: NSEND, NMV
       LOGICAL :: DUMP, FL
 C
+      NSEND=NX*NY*NZ
       IF (.NOT.READY) THEN
          IF ((IXC.NE.0).OR.(IYC.NE.0)) THEN
             WRITE(6,*)'FLUXES: Non-periodic horizontal boundaries'
@@ -110,7 +111,6 @@ C
          ENDIF
          NF=5
          IF (LMAG) NF=8
-         NSEND=NX*NY*NZ
          TBG(1)=TB
          IF (NPE.GT.1) THEN
 C           TB is the bottom ranks' value on a restart (3dmhd.f:354-356)

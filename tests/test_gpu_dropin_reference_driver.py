@@ -8,8 +8,9 @@ the reference's set-up supports).  Run under MPI on 4 ranks, the shim
 gathers the slabs onto rank 0, which steps on the GPU; DT and UMACH are
 broadcast and the slabs scattered back on dump steps.  The files it writes
 must be the reference CPU build's own files byte for byte: every rank's
-rast3d.dat0.NNNN, rast3d.par and the rast3d.lis log (fresh run), and the
-restart from the reference's dump 1 continues as the uninterrupted run.
+rast3d.dat0.NNNN, rast3d.par and the rast3d.lis log (fresh run; restart of
+the hydrodynamic deck).  The reference's restart of the MHD deck is NaN;
+the GPU step reports it and stops.
 
 The executables are built where the reference sources exist; the tests
 skip with that reason elsewhere."""
@@ -65,8 +66,11 @@ def test_reference_driver_with_gpu_step_writes_the_reference_files(tmp_path, nam
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", NAMES)
-def test_reference_driver_with_gpu_step_restarts_from_the_reference_dump(tmp_path, name):
+def test_reference_driver_with_gpu_step_restarts_from_the_reference_dump(tmp_path):
+    """The hydrodynamic deck's restart from the reference's dump 1 writes the
+    reference's own restart files, file for file, and continues as the
+    uninterrupted run."""
+    name = "drvh12"
     print("case reference driver + GPU STEP, restart", name, flush=True)
     _, c = deck(name)
     for f in os.listdir(os.path.join(GOLD, name)):
@@ -74,14 +78,40 @@ def test_reference_driver_with_gpu_step_restarts_from_the_reference_dump(tmp_pat
             shutil.copy(os.path.join(GOLD, name, f), tmp_path)
     r = run(name, tmp_path, int(c["nsteps"]) - 1, nstart=1)
     assert r.returncode == 0, r.stdout + r.stderr
-    lay, lmag = layout(c), name == "drv12"
+    lay = layout(c)
     for k in (1, 2):
-        a = dio().read_dump(os.path.join(tmp_path, "rast3d"), k, lay, lmag)
-        b = dio().read_dump(os.path.join(GOLD, name, "rast3d"), k + 1, lay, lmag)
+        a = dio().read_dump(os.path.join(tmp_path, "rast3d"), k, lay, False)
+        b = dio().read_dump(os.path.join(GOLD, name, "rast3d"), k + 1, lay, False)
         for f in a:
             assert np.array_equal(a[f], b[f]), (k, f)
-    if name == "drvh12":
-        assert same_files(os.path.join(GOLD, name + "_restart"), tmp_path) == []
+    assert same_files(os.path.join(GOLD, name + "_restart"), tmp_path) == []
+
+
+def test_reference_mhd_restart_is_not_finite():
+    """The reference's own restart of the MHD deck (drv12_restart, written by
+    its CPU build) is NaN: a restart reads interiors only, so the physical z
+    ghost planes stay zero, 1/rho is infinite there and the boundary planes
+    of BX, BY turn non-finite in the first step (BZ = 0 times an infinite
+    velocity, 3dmhdsub.f:2228-2240); the reference carries on."""
+    _, c = deck("drv12")
+    a = dio().read_dump(os.path.join(GOLD, "drv12_restart", "rast3d"), 1, layout(c), True)
+    assert all(np.isnan(a[f]).any() for f in ("bx", "by"))
+
+
+@pytest.mark.gpu
+def test_reference_driver_with_gpu_step_stops_the_nonfinite_mhd_restart(tmp_path):
+    """The same restart with the GPU step: the non-finite CFL limit is
+    reported (MHD_ERR_NONFINITE) and every rank stops, instead of writing
+    NaN dumps."""
+    print("case reference driver + GPU STEP, MHD restart", flush=True)
+    _, c = deck("drv12")
+    for f in os.listdir(os.path.join(GOLD, "drv12")):
+        if not f.endswith(".lis"):
+            shutil.copy(os.path.join(GOLD, "drv12", f), tmp_path)
+    r = run("drv12", tmp_path, int(c["nsteps"]) - 1, nstart=1)
+    assert "non-finite CFL limit or Mach number" in r.stdout, r.stdout + r.stderr
+    a = dio().read_dump(os.path.join(tmp_path, "rast3d"), 1, layout(c), True)
+    assert all(np.isfinite(a[f]).all() for f in a)  # dump 1 is the input: nothing was written
 
 
 @pytest.mark.gpu
