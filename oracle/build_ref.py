@@ -85,7 +85,7 @@ def build(name: str, cfg: dict | None = None, main_exe: bool = False) -> str:
     d = os.path.join(REF_OUT, name)
     os.makedirs(d, exist_ok=True)
     mkparam.write_config(d, cfg["params"], par=cfg["par"], ipar=cfg.get("ipar"),
-                         env_steps=main_exe)
+                         env_steps=main_exe, kappa=cfg.get("kappa"))
     write_driver_blocks(d)
     links = {"3dmhdsub.f": os.path.join(REF_SRC, "3dmhdsub.f"),
              "refharness.f": os.path.join(HERE, "refharness.f")}

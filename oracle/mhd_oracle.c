@@ -22,8 +22,17 @@
  * The top/bottom z ghost planes are never refreshed (COMM_MPI has no
  * neighbour there), exactly as in the reference.
  *
- * Out of scope (rejected by orc_check): density diffusion ID!=0, LREM,
- * flux relaxation RLAX>0, ITC=2/3/4 and plume boundary TP!=0, LSHR,
+ * Optional branches (3dmhdsub.f): density diffusion ID=1/2 (1301-1371),
+ * Rempel conduction LREM (1536-1581, BCON 2836, 2862, 2935), flux
+ * relaxation RLAX>0 with a fixed-T top and fixed-flux bottom (1615-1780),
+ * embedded heat loss ITC=2 (1783-1806), the plume boundary TP with ITC=0/1
+ * (2790-2827), ITC=3/4 (2828-2868) and shear boundaries LSHR (1493, 1903-2101,
+ * 2751-2770, 2885-2893).  HORIZONTAL_MEAN follows the reference's summation
+ * order for its NPEY slabs, and the relaxation integral its NPEZ ranks.
+ * Rejected by orc_check: the relaxation with a fixed-flux top and fixed-T
+ * bottom (its integral reads a value the reference never sets on the bottom
+ * rank, FRADM(NZ)), other relaxation boundary pairs (the reference stops),
+ * LSHR together with LREM or RLAX (the reference then uses a stale TTM), and
  * non-periodic horizontal boundaries.
  */
 #include <math.h>
@@ -45,9 +54,112 @@ int orc_check(const orc_state *s)
 {
     if (s->nx < 5 || s->ny < 5 || s->nz < 6) return 1; /* one-sided BX stencils reach K+3 */
     if (s->izc != 0 && s->izc != 1) return 2;          /* BCON: 'Invalid IZCON' (sub:2942) */
-    if (s->itc != 0 && s->itc != 1) return 3;          /* ITC 2/3/4 out of scope */
-    if (s->tp != 0.0) return 4;                        /* plume boundary out of scope */
+    if (s->itc < 0 || s->itc > 4) return 3;
+    if (s->id < 0 || s->id > 2) return 4;
+    if (s->rlax > 0.0 && !(s->itc == 0 && s->izc == 1)) return 5;
+    if (s->lshr && (s->lrem || s->rlax > 0.0)) return 6;
+    if (s->npey < 1 || (s->ny - 2) % s->npey || s->npez < 1 || (s->nz - 2) % s->npez) return 7;
     return 0;
+}
+
+/* HORIZONTAL_MEAN of plane k (sub:3143-3223, NGRID=0 branch) as the
+ * reference forms it on NPEY ranks: each rank sums its rows J (outer) and
+ * points I (inner) sequentially and divides by FLOAT(NPX*NPY) (a float32
+ * value, exact here); rank 0 of the row adds the NPEY quotients in rank order. */
+static double hmean_ref(const orc_state *s, const double *v, int k)
+{
+    const int nx = s->nx, ny = s->ny;
+    const int nry = (ny - 2) / s->npey;
+    const double cnt = (double)(float)((nx - 2) * (ny - 2));
+    double varm = 0.0;
+    for (int p = 0; p < s->npey; p++) {
+        double w = 0.0;
+        for (int j = 2 + p * nry; j <= 1 + (p + 1) * nry; j++)
+            for (int i = 2; i <= nx - 1; i++)
+                w = w + A(v, i, j, k);
+        w = w / cnt;
+        varm = p == 0 ? w : varm + w;
+    }
+    return varm;
+}
+
+/* Flux relaxation (sub:1615-1780), fixed-T top and fixed-flux bottom: the
+ * saved factor TFAC follows the bottom boundary's mean temperature, and FT
+ * gains RLAX*TFAC*CORRECT(K), the vertical integral of the flux deficit
+ * ALPHA, formed per rank of the NPEZ-rank decomposition and offset by the
+ * end values of the ranks above.  W1 = TT - TTM; RO holds 1/rho. */
+static void relax(orc_state *s, const double *ttm)
+{
+    const int nx = s->nx, ny = s->ny, nz = s->nz;
+    const double *RW = s->rw, *UU = s->uu, *VV = s->vv, *WW = s->ww, *W1 = s->ww1;
+    double *W2 = s->ww2, *FT = s->ft;
+    const int nrz = (nz - 2) / s->npez, nzl = nrz + 2;
+    double *fconm = calloc((size_t)nz + 1, sizeof(double)), *fradm = calloc((size_t)nz + 1, sizeof(double));
+    double *alpha = calloc((size_t)nzl + 1, sizeof(double)), *corr = calloc((size_t)nz + 1, sizeof(double));
+    double *endval = calloc((size_t)s->npez + 1, sizeof(double));
+    int i, j, k;
+
+    if (s->timt > s->tstart && s->tfac > 1e-4) {
+        /* the bottom rank's boundary-plane mean temperature (sub:1620-1630) */
+        if (ttm[nz - 1 - 1] <= s->ttm_max) {
+            s->tfac = s->tfac - s->dt / (3.0 * s->toff);
+        } else {
+            s->ttm_max = ttm[nz - 1 - 1];
+            s->tfac = s->tfac + s->dt / (3.0 * s->toff);
+            s->tfac = fmin(s->tfac, 2.0);
+        }
+        /* mean convective flux (sub:1653-1662) on K=2..NZ */
+        for (k = 2; k <= nz; k++)
+            for (j = 2; j <= ny - 1; j++)
+                for (i = 2; i <= nx - 1; i++)
+                    A(W2, i, j, k) = -(A(RW, i, j, k) * (2.5 * A(W1, i, j, k)
+                        + 0.5 * (A(UU, i, j, k) * A(UU, i, j, k) + A(VV, i, j, k) * A(VV, i, j, k)
+                                 + A(WW, i, j, k) * A(WW, i, j, k))));
+        for (k = 2; k <= nz; k++) fconm[k] = hmean_ref(s, W2, k);
+        /* mean radiative flux (sub:1665-1677); at a rank's last plane it is
+         * the next rank's first, so the global centred value */
+        for (k = 3; k <= nz - 2; k++)
+            fradm[k] = (ttm[k + 1 - 1] - ttm[k - 1 - 1]) * s->hz * MX(s->dzzdz, k) * MX(s->rkapa, k) / s->repr;
+        fradm[2] = (-3.0 * ttm[2 - 1] + 4.0 * ttm[3 - 1] - ttm[4 - 1]) * s->hz * MX(s->dzzdz, 2)
+                   * MX(s->rkapa, 2) / s->repr;
+        fradm[nz - 1] = (3.0 * ttm[nz - 1 - 1] - 4.0 * ttm[nz - 2 - 1] + ttm[nz - 3 - 1]) * s->hz
+                        * MX(s->dzzdz, nz - 1) * MX(s->rkapa, nz - 1) / s->repr;
+        const double ftot = s->lrem ? (double)(8.07f * 0.4f) * s->theta : s->theta / s->repr;
+        /* per rank: ALPHA and the running integral CORRECT (sub:1693-1705);
+         * the bottom rank's last plane (the ghost plane) is never used */
+        for (int pz = 0; pz < s->npez; pz++) {
+            const int kl_end = pz == s->npez - 1 ? nzl - 1 : nzl;
+            double c = 0.0;
+            for (int kl = 2; kl <= kl_end; kl++) {
+                const int kg = pz * nrz + kl;
+                alpha[kl] = 1.0 - (fconm[kg] + fradm[kg]) / ftot;
+                alpha[kl] = alpha[kl] / (1.0 + 4.0 * fabs(alpha[kl]));
+                if (kl == 2) c = 0.0;
+                else c = c + 0.5 * (alpha[kl] + alpha[kl - 1]) * (MX(s->zee, kg) - MX(s->zee, kg - 1));
+                if (kl <= nzl - 1) corr[kg] = c;
+            }
+            endval[pz] = c;
+        }
+        /* offsets: the end values of the ranks above, added in rank order (sub:1733-1743) */
+        for (int pz = 0; pz < s->npez; pz++) {
+            double sum = 0.0;
+            for (int ipe = 1; ipe <= pz; ipe++) sum = sum + endval[ipe - 1];
+            for (int kl = 2; kl <= nzl - 1; kl++) corr[pz * nrz + kl] = corr[pz * nrz + kl] + sum;
+        }
+    } else {
+        for (k = 2; k <= nz - 1; k++) corr[k] = 0.0;
+    }
+    const double rt = s->rlax * s->tfac;
+    for (k = 2; k <= nz - 1; k++)
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(FT, i, j, k) = A(FT, i, j, k) + rt * corr[k];
+    s->icall = s->icall + 1;
+    free(fconm);
+    free(fradm);
+    free(alpha);
+    free(corr);
+    free(endval);
 }
 
 /* ---------------------------------------------------------------------------
@@ -103,6 +215,77 @@ void orc_fluxes(orc_state *s)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
                 A(FR, i, j, k) = A(FR, i, j, k) - A(W1, i, j, k);
+
+    /* density perturbations: ghost planes of RO mirrored about the boundary
+     * planes (they stay so in /BIG/), W1 = RO - ROM (sub:1301-1318) */
+    if (s->id != 0 || s->lrem) {
+        for (j = 1; j <= ny; j++)
+            for (i = 1; i <= nx; i++) {
+                A(RO, i, j, 1) = A(RO, i, j, 3);
+                A(RO, i, j, nz) = A(RO, i, j, nz - 2);
+            }
+        double *rom = malloc(sizeof(double) * (size_t)nz);
+        for (k = 1; k <= nz; k++) rom[k - 1] = hmean_ref(s, RO, k);
+        for (k = 1; k <= nz; k++)
+            for (j = 1; j <= ny; j++)
+                for (i = 1; i <= nx; i++)
+                    A(W1, i, j, k) = A(RO, i, j, k) - rom[k - 1];
+        if (s->id != 0) {
+            /* diffusion of the density perturbations, weighted by 1/ROM (ID=1)
+             * or confined to the boundaries (ID=2) (sub:1319-1371) */
+            double *wwz = malloc(sizeof(double) * (size_t)nz);
+            if (s->id == 1) {
+                for (k = 1; k <= nz; k++) wwz[k - 1] = 1.0 / rom[k - 1];
+            } else {
+                const double sgm = (double)0.1f;
+                const double cln = -((double)(4.0f * logf(2.0f)) / sgm / sgm);
+                for (k = 1; k <= nz; k++) {
+                    const double z = MX(s->zee, k), zm = z - s->zmax;
+                    wwz[k - 1] = exp(cln * (z * z)) + exp(cln * (zm * zm));
+                }
+            }
+            if (s->dh != 0.0) {
+                for (k = 2; k <= nz - 1; k++)
+                    for (j = 2; j <= ny - 1; j++)
+                        for (i = 2; i <= nx - 1; i++)
+                            A(W2, i, j, k) = (A(W1, i + 1, j, k) - A(W1, i - 1, j, k)) * HX * MX(D2XXDX2, i)
+                                + (A(W1, i + 1, j, k) - 2.0 * A(W1, i, j, k) + A(W1, i - 1, j, k))
+                                  * H2X * MX(DXXDX, i) * MX(DXXDX, i);
+                for (k = 2; k <= nz - 1; k++)
+                    for (j = 2; j <= ny - 1; j++) {
+                        tmpy1 = HY * MX(D2YYDY2, j);
+                        tmpy2 = H2Y * MX(DYYDY, j) * MX(DYYDY, j);
+                        for (i = 2; i <= nx - 1; i++)
+                            A(W3, i, j, k) = (A(W1, i, j + 1, k) - A(W1, i, j - 1, k)) * tmpy1
+                                + (A(W1, i, j + 1, k) - 2.0 * A(W1, i, j, k) + A(W1, i, j - 1, k)) * tmpy2;
+                    }
+                for (k = 2; k <= nz - 1; k++) {
+                    tmp = s->dh * ORE * wwz[k - 1];
+                    for (j = 2; j <= ny - 1; j++)
+                        for (i = 2; i <= nx - 1; i++)
+                            A(FR, i, j, k) = A(FR, i, j, k) + (A(W2, i, j, k) + A(W3, i, j, k)) * tmp;
+                }
+            }
+            if (s->dv != 0.0) {
+                for (k = 2; k <= nz - 1; k++) {
+                    tmpz1 = HZ * MX(D2ZZDZ2, k);
+                    tmpz2 = H2Z * MX(DZZDZ, k) * MX(DZZDZ, k);
+                    for (j = 2; j <= ny - 1; j++)
+                        for (i = 2; i <= nx - 1; i++)
+                            A(W2, i, j, k) = (A(W1, i, j, k + 1) - A(W1, i, j, k - 1)) * tmpz1
+                                + (A(W1, i, j, k + 1) - 2.0 * A(W1, i, j, k) + A(W1, i, j, k - 1)) * tmpz2;
+                }
+                for (k = 2; k <= nz - 1; k++) {
+                    tmp = s->dv * ORE * wwz[k - 1];
+                    for (j = 2; j <= ny - 1; j++)
+                        for (i = 2; i <= nx - 1; i++)
+                            A(FR, i, j, k) = A(FR, i, j, k) + A(W2, i, j, k) * tmp;
+                }
+            }
+            free(wwz);
+        }
+        free(rom);
+    }
 
     /* buoyancy (sub:1376-1380) */
     for (k = 2; k <= nz - 1; k++)
@@ -185,19 +368,70 @@ void orc_fluxes(orc_state *s)
     }
 
     /* temperature advection (sub:1495-1506); the leading unary minus negates
-     * the whole product, which is bit-identical to negating UU first. */
-    for (k = 2; k <= nz - 1; k++) {
-        tmpz = HZ * MX(DZZDZ, k);
-        for (j = 2; j <= ny - 1; j++) {
-            tmpy = HY * MX(DYYDY, j);
-            for (i = 2; i <= nx - 1; i++) {
-                A(FT, i, j, k) = -(A(UU, i, j, k) * (A(TT, i + 1, j, k) - A(TT, i - 1, j, k)) * HX * MX(DXXDX, i));
-                A(FT, i, j, k) = A(FT, i, j, k) - A(VV, i, j, k) * (A(TT, i, j + 1, k) - A(TT, i, j - 1, k)) * tmpy;
-                A(FT, i, j, k) = A(FT, i, j, k) - A(WW, i, j, k) * (A(TT, i, j, k + 1) - A(TT, i, j, k - 1)) * tmpz;
+     * the whole product, which is bit-identical to negating UU first.  With
+     * LSHR there is none and FT is not reset: it keeps the previous call's
+     * values and the terms below add to them (sub:1493). */
+    double *ttm = NULL;
+    if (!s->lshr) {
+        for (k = 2; k <= nz - 1; k++) {
+            tmpz = HZ * MX(DZZDZ, k);
+            for (j = 2; j <= ny - 1; j++) {
+                tmpy = HY * MX(DYYDY, j);
+                for (i = 2; i <= nx - 1; i++) {
+                    A(FT, i, j, k) = -(A(UU, i, j, k) * (A(TT, i + 1, j, k) - A(TT, i - 1, j, k)) * HX * MX(DXXDX, i));
+                    A(FT, i, j, k) = A(FT, i, j, k) - A(VV, i, j, k) * (A(TT, i, j + 1, k) - A(TT, i, j - 1, k)) * tmpy;
+                    A(FT, i, j, k) = A(FT, i, j, k) - A(WW, i, j, k) * (A(TT, i, j, k + 1) - A(TT, i, j, k - 1)) * tmpz;
+                }
             }
+        }
+        /* temperature perturbations W1 = TT - TTM (sub:1519-1532) */
+        if (s->rlax > 0.0 || s->lrem) {
+            ttm = malloc(sizeof(double) * (size_t)nz);
+            for (k = 1; k <= nz; k++) ttm[k - 1] = hmean_ref(s, TT, k);
+            for (k = 1; k <= nz; k++)
+                for (j = 1; j <= ny; j++)
+                    for (i = 1; i <= nx; i++)
+                        A(W1, i, j, k) = A(TT, i, j, k) - ttm[k - 1];
         }
     }
 
+    if (s->lrem) {
+        /* conduction of the temperature perturbations only, and the
+         * radiative heating of the mean profile (sub:1536-1581) */
+        tmp = OCV / REPR;
+        for (k = 2; k <= nz - 1; k++) {
+            tmpz1 = HZ * MX(D2ZZDZ2, k);
+            tmpz2 = H2Z * MX(DZZDZ, k) * MX(DZZDZ, k);
+            for (j = 2; j <= ny - 1; j++) {
+                tmpy1 = HY * MX(D2YYDY2, j);
+                tmpy2 = H2Y * MX(DYYDY, j) * MX(DYYDY, j);
+                for (i = 2; i <= nx - 1; i++) {
+                    const double w0 = A(W1, i, j, k);
+                    A(FT, i, j, k) = A(FT, i, j, k)
+                        + ((A(W1, i + 1, j, k) - A(W1, i - 1, j, k)) * HX * MX(D2XXDX2, i)
+                           + (A(W1, i + 1, j, k) - 2.0 * w0 + A(W1, i - 1, j, k)) * H2X * MX(DXXDX, i) * MX(DXXDX, i))
+                          * A(RO, i, j, k) * tmp;
+                    A(FT, i, j, k) = A(FT, i, j, k)
+                        + ((A(W1, i, j + 1, k) - A(W1, i, j - 1, k)) * tmpy1
+                           + (A(W1, i, j + 1, k) - 2.0 * w0 + A(W1, i, j - 1, k)) * tmpy2)
+                          * A(RO, i, j, k) * tmp;
+                    A(FT, i, j, k) = A(FT, i, j, k)
+                        + ((A(W1, i, j, k + 1) - A(W1, i, j, k - 1)) * tmpz1
+                           + (A(W1, i, j, k + 1) - 2.0 * w0 + A(W1, i, j, k - 1)) * tmpz2)
+                          * A(RO, i, j, k) * tmp;
+                }
+            }
+        }
+        for (k = 2; k <= nz - 1; k++) {
+            const double tp1 = ttm[k], t0 = ttm[k - 1], tm1 = ttm[k - 2];
+            const double hrad = ((tp1 - 2.0 * t0 + tm1) * H2Z * MX(DZZDZ, k) * MX(DZZDZ, k)
+                                 + (tp1 - tm1) * HZ * MX(D2ZZDZ2, k)) * MX(RKAPA, k)
+                                + (tp1 - tm1) * HZ * MX(DZZDZ, k) * MX(DKAPA, k);
+            for (j = 2; j <= ny - 1; j++)
+                for (i = 2; i <= nx - 1; i++)
+                    A(FT, i, j, k) = A(FT, i, j, k) + A(RO, i, j, k) * tmp * hrad;
+        }
+    } else {
     /* thermal conduction with depth-dependent kappa (sub:1584-1610) */
     tmp = OCV / REPR;
     for (k = 2; k <= nz - 1; k++) {
@@ -224,6 +458,31 @@ void orc_fluxes(orc_state *s)
                     + (A(TT, i, j, k + 1) - A(TT, i, j, k - 1)) * HZ * MX(DZZDZ, k) * A(RO, i, j, k) * tmp * MX(DKAPA, k);
             }
         }
+    }
+    }
+
+    if (s->rlax > 0.0) relax(s, ttm);
+    free(ttm);
+
+    /* embedded heat loss, ramping up to TP with a tanh in time when TC != 0
+     * (sub:1783-1806) */
+    if (s->itc == 2) {
+        const double cln = -((double)(4.0f * logf(2.0f)) / s->hh / s->hh);
+        const double HH = s->hh, TP = s->tp;
+        const double ramp = 1.0 + tanh((double)logf(3.0f) / s->qfh * (s->timt - s->tc));
+        for (k = 2; k <= nz - 1; k++)
+            for (j = 2; j <= ny - 1; j++)
+                for (i = 2; i <= nx - 1; i++) {
+                    const double dx = MX(s->exx, i) - s->xp, dy = MX(s->wyy, j) - s->yp, dzz = MX(s->zee, k) - s->zp;
+                    if (s->tc != 0.0)
+                        A(FT, i, j, k) = A(FT, i, j, k)
+                            - TP * OCV * 0.5 * ramp * A(RO, i, j, k) * exp(cln * (dx * dx)) / HH
+                              * exp(cln * (dy * dy)) / HH * exp(cln * (dzz * dzz)) / HH;
+                    else
+                        A(FT, i, j, k) = A(FT, i, j, k)
+                            - TP * OCV * A(RO, i, j, k) * exp(cln * (dx * dx)) / HH
+                              * exp(cln * (dy * dy)) / HH * exp(cln * (dzz * dzz)) / HH;
+                }
     }
 
     /* viscous Laplacians with the 4/3 factor on the normal component (sub:1818-1886) */
@@ -292,6 +551,9 @@ void orc_fluxes(orc_state *s)
         for (j = 1; j <= ny; j++)
             for (i = 2; i <= nx - 1; i++)
                 A(W2, i, j, k) = (A(VV, i + 1, j, k) - A(VV, i - 1, j, k)) * HX * MX(DXXDX, i);
+    /* (with LSHR the heating terms and the scratch they need are skipped:
+     * sub:1903, 1946, 1999, 2050, 2084) */
+    if (!s->lshr) {
     for (k = 2; k <= nz - 1; k++)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
@@ -307,6 +569,7 @@ void orc_fluxes(orc_state *s)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
                 A(FT, i, j, k) = A(FT, i, j, k) - OCV * A(W1, i, j, k) * A(TT, i, j, k);
+    }
     tmp = C13 * ORE;
     for (k = 2; k <= nz - 1; k++)
         for (j = 2; j <= ny - 1; j++)
@@ -316,6 +579,7 @@ void orc_fluxes(orc_state *s)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
                 A(FV, i, j, k) = A(FV, i, j, k) + tmp * (A(W1, i, j + 1, k) - A(W1, i, j - 1, k)) * HY * MX(DYYDY, j);
+    if (!s->lshr) {
     for (k = 2; k <= nz - 1; k++)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
@@ -343,10 +607,12 @@ void orc_fluxes(orc_state *s)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
                 A(W1, i, j, k) = (A(UU, i + 1, j, k) - A(UU, i - 1, j, k)) * HX * MX(DXXDX, i);
+    }
     for (k = 2; k <= nz - 1; k++)
         for (j = 1; j <= ny; j++)
             for (i = 1; i <= nx; i++)
                 A(W3, i, j, k) = (A(WW, i, j, k + 1) - A(WW, i, j, k - 1)) * HZ * MX(DZZDZ, k);
+    if (!s->lshr) {
     tmp = C43 * OCV * ORE;
     for (k = 2; k <= nz - 1; k++)
         for (j = 2; j <= ny - 1; j++)
@@ -358,6 +624,7 @@ void orc_fluxes(orc_state *s)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
                 A(FT, i, j, k) = A(FT, i, j, k) - OCV * A(W3, i, j, k) * A(TT, i, j, k);
+    }
     tmp = C13 * ORE;
     for (k = 2; k <= nz - 1; k++)
         for (j = 2; j <= ny - 1; j++)
@@ -375,6 +642,7 @@ void orc_fluxes(orc_state *s)
         for (j = 1; j <= ny; j++)
             for (i = 2; i <= nx - 1; i++)
                 A(W2, i, j, k) = (A(VV, i, j, k + 1) - A(VV, i, j, k - 1)) * HZ * MX(DZZDZ, k);
+    if (!s->lshr) {
     for (k = 2; k <= nz - 1; k++)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
@@ -386,6 +654,7 @@ void orc_fluxes(orc_state *s)
                 A(FT, i, j, k) = A(FT, i, j, k)
                     + (A(W1, i, j, k) * A(W1, i, j, k) + A(W2, i, j, k) * A(W2, i, j, k)
                        + A(W3, i, j, k) * A(W3, i, j, k) + 2.0 * A(W2, i, j, k) * A(W3, i, j, k)) * A(RO, i, j, k) * tmp;
+    }
     tmp = C13 * ORE;
     for (k = 2; k <= nz - 1; k++)
         for (j = 2; j <= ny - 1; j++)
@@ -393,6 +662,7 @@ void orc_fluxes(orc_state *s)
                 A(FW, i, j, k) = A(FW, i, j, k)
                     + tmp * ((A(W1, i + 1, j, k) - A(W1, i - 1, j, k)) * HX * MX(DXXDX, i)
                              + (A(W2, i, j + 1, k) - A(W2, i, j - 1, k)) * HY * MX(DYYDY, j));
+    if (!s->lshr) {
     for (k = 2; k <= nz - 1; k++)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
@@ -402,6 +672,7 @@ void orc_fluxes(orc_state *s)
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
                 A(FT, i, j, k) = A(FT, i, j, k) + 2.0 * A(W1, i, j, k) * A(W3, i, j, k) * A(RO, i, j, k) * tmp;
+    }
 
     /* Coriolis, 2 Omega = (OMX, 0, OMZ) (sub:2106-2125) */
     if (s->lrot) {
@@ -557,28 +828,71 @@ void orc_bcon(orc_state *s)
     int i, j;
     double dz;
     const int kt = 2, kb = nz - 1;
+    /* shear boundaries: RRT, RRB, RPP (sub:2751-2756) */
+    const double rrt = 1.0, rrb = -1.0, rpp = 0.0;
 
     /* top (sub:2771-2868) */
-    for (j = 2; j <= ny - 1; j++)
-        for (i = 2; i <= nx - 1; i++)
-            A(RU, i, j, kt) = A(RO, i, j, kt)
-                * (C43 * A(RU, i, j, kt + 1) / A(RO, i, j, kt + 1) - C13 * A(RU, i, j, kt + 2) / A(RO, i, j, kt + 2));
-    for (j = 2; j <= ny - 1; j++)
-        for (i = 2; i <= nx - 1; i++)
-            A(RV, i, j, kt) = A(RO, i, j, kt)
-                * (C43 * A(RV, i, j, kt + 1) / A(RO, i, j, kt + 1) - C13 * A(RV, i, j, kt + 2) / A(RO, i, j, kt + 2));
+    if (s->lshr) {
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                A(RU, i, j, kt) = 0.0;
+                A(RV, i, j, kt) = rrt * cos(rpp * s->timt) * A(RO, i, j, kt);
+            }
+    } else {
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(RU, i, j, kt) = A(RO, i, j, kt)
+                    * (C43 * A(RU, i, j, kt + 1) / A(RO, i, j, kt + 1) - C13 * A(RU, i, j, kt + 2) / A(RO, i, j, kt + 2));
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(RV, i, j, kt) = A(RO, i, j, kt)
+                    * (C43 * A(RV, i, j, kt + 1) / A(RO, i, j, kt + 1) - C13 * A(RV, i, j, kt + 2) / A(RO, i, j, kt + 2));
+    }
     for (j = 2; j <= ny - 1; j++)
         for (i = 2; i <= nx - 1; i++)
             A(RW, i, j, kt) = 0.0;
-    if (s->itc == 0) {
-        for (j = 2; j <= ny - 1; j++)
-            for (i = 2; i <= nx - 1; i++)
-                A(TT, i, j, kt) = 1.0;
+    if (s->tp != 0.0 && (s->itc == 0 || s->itc == 1)) {
+        /* Gaussian top temperature (ITC=0) or flux (ITC=1) of full width at
+         * half maximum HH, relaxing from 1 to TP over TC (sub:2790-2827) */
+        const double tpr = s->tc != 0.0 ? s->tp + (1.0 - s->tp) * exp(-(s->timt / s->tc)) : s->tp;
+        const double cln = -((double)(4.0f * logf(2.0f)) / s->hh / s->hh);
+        const double HH = s->hh;
+        if (s->itc == 0) {
+            for (j = 2; j <= ny - 1; j++)
+                for (i = 2; i <= nx - 1; i++) {
+                    const double dx = MX(s->exx, i) - s->xp, dy = MX(s->wyy, j) - s->yp;
+                    A(TT, i, j, kt) = 1.0 - (1.0 - tpr) * exp(cln * (dx * dx)) / HH * exp(cln * (dy * dy)) / HH;
+                }
+        } else {
+            dz = s->dzf / MX(s->dzzdz, kt);
+            for (j = 2; j <= ny - 1; j++)
+                for (i = 2; i <= nx - 1; i++) {
+                    const double dx = MX(s->exx, i) - s->xp, dy = MX(s->wyy, j) - s->yp;
+                    A(TT, i, j, kt) = C43 * A(TT, i, j, kt + 1) - C13 * A(TT, i, j, kt + 2)
+                        - C23 * dz * (s->theta - (s->theta - tpr) * exp(cln * (dx * dx)) / HH * exp(cln * (dy * dy)) / HH);
+                }
+        }
     } else {
-        dz = s->dzf / MX(s->dzzdz, kt);
-        for (j = 2; j <= ny - 1; j++)
-            for (i = 2; i <= nx - 1; i++)
-                A(TT, i, j, kt) = C43 * A(TT, i, j, kt + 1) - C13 * A(TT, i, j, kt + 2) - C23 * dz * s->theta;
+        if (s->itc == 0 || s->itc == 2 || s->itc == 4) {
+            /* fixed T (TU with LREM); ITC=4: TP cooler on I <= NX/2 */
+            for (j = 2; j <= ny - 1; j++)
+                for (i = 2; i <= nx - 1; i++) {
+                    if (s->lrem) {
+                        A(TT, i, j, kt) = s->tu;
+                    } else {
+                        A(TT, i, j, kt) = 1.0;
+                        if (s->itc == 4 && i <= nx / 2) A(TT, i, j, kt) = 1.0 - s->tp;
+                    }
+                }
+        }
+        if (s->itc == 1) {
+            dz = s->dzf / MX(s->dzzdz, kt);
+            for (j = 2; j <= ny - 1; j++)
+                for (i = 2; i <= nx - 1; i++)
+                    A(TT, i, j, kt) = s->lrem ? C43 * A(TT, i, j, kt + 1) - C13 * A(TT, i, j, kt + 2) + s->dztu
+                                              : C43 * A(TT, i, j, kt + 1) - C13 * A(TT, i, j, kt + 2) - C23 * dz * s->theta;
+        }
+        /* ITC=3: the top temperature is not set */
     }
     if (s->lmag && s->ibc == 0)
         for (j = 2; j <= ny - 1; j++)
@@ -586,14 +900,22 @@ void orc_bcon(orc_state *s)
                 A(BZ, i, j, kt) = 0.0;
 
     /* bottom (sub:2873-2955) */
-    for (j = 2; j <= ny - 1; j++)
-        for (i = 2; i <= nx - 1; i++)
-            A(RU, i, j, kb) = A(RO, i, j, kb)
-                * (C43 * A(RU, i, j, kb - 1) / A(RO, i, j, kb - 1) - C13 * A(RU, i, j, kb - 2) / A(RO, i, j, kb - 2));
-    for (j = 2; j <= ny - 1; j++)
-        for (i = 2; i <= nx - 1; i++)
-            A(RV, i, j, kb) = A(RO, i, j, kb)
-                * (C43 * A(RV, i, j, kb - 1) / A(RO, i, j, kb - 1) - C13 * A(RV, i, j, kb - 2) / A(RO, i, j, kb - 2));
+    if (s->lshr) {
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++) {
+                A(RU, i, j, kb) = 0.0;
+                A(RV, i, j, kb) = rrb * cos(rpp * s->timt) * A(RO, i, j, kb);
+            }
+    } else {
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(RU, i, j, kb) = A(RO, i, j, kb)
+                    * (C43 * A(RU, i, j, kb - 1) / A(RO, i, j, kb - 1) - C13 * A(RU, i, j, kb - 2) / A(RO, i, j, kb - 2));
+        for (j = 2; j <= ny - 1; j++)
+            for (i = 2; i <= nx - 1; i++)
+                A(RV, i, j, kb) = A(RO, i, j, kb)
+                    * (C43 * A(RV, i, j, kb - 1) / A(RO, i, j, kb - 1) - C13 * A(RV, i, j, kb - 2) / A(RO, i, j, kb - 2));
+    }
     for (j = 2; j <= ny - 1; j++)
         for (i = 2; i <= nx - 1; i++)
             A(RW, i, j, kb) = 0.0;
@@ -605,8 +927,9 @@ void orc_bcon(orc_state *s)
         dz = s->dzf / MX(s->dzzdz, kb);
         for (j = 2; j <= ny - 1; j++)
             for (i = 2; i <= nx - 1; i++)
-                A(TT, i, j, kb) = C43 * A(TT, i, j, kb - 1) - C13 * A(TT, i, j, kb - 2)
-                                  + C23 * dz * s->theta / MX(s->rkapa, kb);
+                A(TT, i, j, kb) = s->lrem ? C43 * A(TT, i, j, kb - 1) - C13 * A(TT, i, j, kb - 2) + s->dztb
+                                          : C43 * A(TT, i, j, kb - 1) - C13 * A(TT, i, j, kb - 2)
+                                            + C23 * dz * s->theta / MX(s->rkapa, kb);
     }
     if (s->lmag && s->ibc == 0)
         for (j = 2; j <= ny - 1; j++)
@@ -695,7 +1018,10 @@ void orc_cfl(orc_state *s)
                 if (MX(s->ddy, j) < dd) dd = MX(s->ddy, j);
                 if (MX(s->ddz, k) < dd) dd = MX(s->ddz, k);
                 A(W1, i, j, k) = dd / sqrt(A(UU, i, j, k));
-                A(W2, i, j, k) = 0.5 * dd * dd * s->repr * s->cv * A(RO, i, j, k) / MX(s->rkapa, k);
+                /* with LREM the conductivity is 1 + RKAPA: a constant part and the
+                 * depth-dependent one (sub:844-852) */
+                A(W2, i, j, k) = 0.5 * dd * dd * s->repr * s->cv * A(RO, i, j, k)
+                                 / (s->lrem ? 1.0 + MX(s->rkapa, k) : MX(s->rkapa, k));
                 A(W3, i, j, k) = 0.375 * dd * dd * s->re * A(RO, i, j, k);
                 if (s->lmag) A(VV, i, j, k) = 0.5 * dd * dd * s->rm;
                 if (A(W1, i, j, k) < wmin[0]) wmin[0] = A(W1, i, j, k);

@@ -43,6 +43,23 @@ typedef struct orc_state {
     double *ww1, *ww2, *ww3;
     double *bx, *by, *bz;
     double *zbx, *zby, *zbz;
+    /* Optional branches of FLUXES and BCON (all zero: the main path).
+     *   id, dh, dv      density diffusion (3dmhdparam.f ID, DH, DV)
+     *   lrem, tu, dztu, dztb   Rempel conduction, /SPECIALBOUND/
+     *   lshr            shear boundaries
+     *   rlax, tstart, toff     flux relaxation, /RELAX/
+     *   xp..hh          /CPER/ (TP above): plume boundary, embedded heat loss
+     *   zmax            /BOUNDS/ ZMAX (ID=2)
+     *   npey, npez      the MPI decomposition whose summation order the
+     *                   horizontal means and the relaxation integral follow
+     *   exx, wyy, zee   /AJACOBI/ coordinates (lengths nx, ny, nz)
+     *   icall, ttm_min, ttm_max, tfac   FLUXES' saved relaxation state
+     *                   (DATA 0, 1D99, -1D99, 1D0) */
+    int id, lrem, lshr, npey, npez, icall;
+    double dh, dv, tu, dztu, dztb, rlax, tstart, toff;
+    double xp, yp, zp, tc, qfh, hh, zmax;
+    double ttm_min, ttm_max, tfac;
+    const double *exx, *wyy, *zee;
 } orc_state;
 
 int orc_check(const orc_state *s);

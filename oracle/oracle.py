@@ -29,6 +29,12 @@ ARRAY_KEYS = ["ru", "rv", "rw", "ro", "tt", "uu", "vv", "ww", "fu", "fv", "fw", 
               "zru", "zrv", "zrw", "zro", "ztt", "ww1", "ww2", "ww3",
               "bx", "by", "bz", "zbx", "zby", "zbz"]
 STATE_KEYS = ["ru", "rv", "rw", "ro", "tt", "bx", "by", "bz"]
+# optional FLUXES/BCON branches and the saved relaxation state (mhd_oracle.h)
+OPT_INT_KEYS = ["id", "lrem", "lshr", "npey", "npez", "icall"]
+OPT_DBL_KEYS = ["dh", "dv", "tu", "dztu", "dztb", "rlax", "tstart", "toff",
+                "xp", "yp", "zp", "tc", "qfh", "hh", "zmax", "ttm_min", "ttm_max", "tfac"]
+OPT_DEFAULTS = {"npey": 1, "npez": 1, "ttm_min": 1e99, "ttm_max": -1e99, "tfac": 1.0}
+COORD_KEYS = ["exx", "wyy", "zee"]
 
 _P = C.POINTER(C.c_double)
 
@@ -38,7 +44,10 @@ class _OrcState(C.Structure):
                 + [(k, C.c_double) for k in DBL_KEYS]
                 + [("cflmin", C.c_double * 4)]
                 + [(k, _P) for k in METRIC_KEYS]
-                + [(k, _P) for k in ARRAY_KEYS])
+                + [(k, _P) for k in ARRAY_KEYS]
+                + [(k, C.c_int) for k in OPT_INT_KEYS]
+                + [(k, C.c_double) for k in OPT_DBL_KEYS]
+                + [(k, _P) for k in COORD_KEYS])
 
 
 def build(force: bool = False) -> str:
@@ -76,7 +85,9 @@ class Oracle:
     """One resident box: scalars in `meta`, /BIG/ arrays in `arrays`."""
 
     def __init__(self, meta: dict, fields: dict):
-        self.meta = {k: meta[k] for k in INT_KEYS + DBL_KEYS if k in meta}
+        self.meta = {k: meta[k] for k in INT_KEYS + DBL_KEYS + OPT_INT_KEYS + OPT_DBL_KEYS if k in meta}
+        for k, v in OPT_DEFAULTS.items():
+            self.meta.setdefault(k, v)
         self.meta.setdefault("nit", 0)
         for k in ("dt", "timt", "timc", "timi", "umach"):
             self.meta.setdefault(k, 0.0)
@@ -86,6 +97,8 @@ class Oracle:
         for k, n in zip(METRIC_KEYS, [nx] * 3 + [ny] * 3 + [nz] * 5):
             if self.metrics[k].shape != (n,):
                 raise ValueError(f"metric {k} has shape {self.metrics[k].shape}, expected ({n},)")
+        self.coords = {k: np.ascontiguousarray(meta[k], dtype=np.float64) if k in meta else np.zeros(n)
+                       for k, n in zip(COORD_KEYS, (nx, ny, nz))}
         self.arrays = {k: np.zeros(self.shape, dtype=np.float64, order="F") for k in ARRAY_KEYS}
         for k in STATE_KEYS:
             if k in fields:
@@ -101,10 +114,12 @@ class Oracle:
 
     def _sync_in(self):
         s = self._s
-        for k in INT_KEYS:
+        for k in INT_KEYS + OPT_INT_KEYS:
             setattr(s, k, int(self.meta.get(k, 0)))
-        for k in DBL_KEYS:
+        for k in DBL_KEYS + OPT_DBL_KEYS:
             setattr(s, k, float(self.meta.get(k, 0.0)))
+        for k in COORD_KEYS:
+            setattr(s, k, _ptr(self.coords[k]))
         for k in METRIC_KEYS:
             setattr(s, k, _ptr(self.metrics[k]))
         for k in ARRAY_KEYS:
@@ -113,8 +128,9 @@ class Oracle:
     def _sync_out(self):
         for k in ("nit",):
             self.meta[k] = getattr(self._s, k)
-        for k in ("dt", "timt", "timc", "timi", "umach"):
+        for k in ("dt", "timt", "timc", "timi", "umach", "ttm_min", "ttm_max", "tfac"):
             self.meta[k] = getattr(self._s, k)
+        self.meta["icall"] = self._s.icall
         self.cflmin = list(self._s.cflmin)
 
     def _call(self, fn):

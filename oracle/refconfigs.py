@@ -89,6 +89,55 @@ CONFIGS = {
     ),
 }
 
+# The optional branches of FLUXES and BCON, one small box each (12x8x8 on
+# 2 x 2 ranks; the flux relaxation on 2 x 3 ranks so that its vertical
+# integral crosses two rank boundaries).
+_HYD = {1: "1.0E02", 2: "1.0E00", 3: "2.0E00", 4: "4.0E00", 20: "1.0E-03"}
+_MAG = {1: "1.5E02", 2: "1.0E00", 3: "2.0E00", 4: "4.0E00", 5: "2.0E00", 6: ANG45,
+        7: "1.2E02", 8: "5.0E01", 20: "1.0E-03", 21: "1.0E-01", 22: "0.3E00", 23: "0.5E00"}
+_BOX = dict(NPX=12, NPY=8, NPZ=8, NPEY=2, NPEZ=2)
+_MAGP = dict(LMAG=".TRUE.", LROT=".TRUE.")
+
+
+def _opt(params, par, seed, ampb="0.0D0", **kw):
+    return dict(params={**_BOX, **params}, par=par, steps=3, dumps=[1, 3], ampu="2.0D-02", ampb=ampb,
+                seed=seed, **kw)
+
+
+OPT_CONFIGS = {
+    # top T prognostic (no BCON branch for ITCON=3)
+    "itc3": _opt(dict(IZCON=1, ITCON=3), _HYD, 31),
+    # top T fixed, TP cooler on the half domain I <= NX/2
+    "itc4": _opt(dict(IZCON=1, ITCON=4, **_MAGP), {**_MAG, 12: "2.0E-01"}, 32, ampb="2.0D-02"),
+    # Gaussian top temperature (plume boundary), steady
+    "plume0": _opt(dict(IZCON=1, ITCON=0), {**_HYD, 12: "5.0E-01", 14: "2.0E00", 15: "1.5E00", 55: "7.0E-01"}, 33),
+    # Gaussian top flux relaxing in time (TC != 0)
+    "plume1": _opt(dict(IZCON=0, ITCON=1, **_MAGP),
+                   {**_MAG, 12: "3.0E00", 13: "5.0E-02", 14: "1.0E00", 15: "2.5E00", 55: "9.0E-01"}, 34,
+                   ampb="2.0D-02"),
+    # embedded heat loss ramping up in time (TC != 0) and without ramp (TC = 0)
+    "loss2": _opt(dict(IZCON=1, ITCON=2),
+                  {**_HYD, 12: "3.0E-01", 13: "2.0E-02", 14: "2.0E00", 15: "2.0E00", 16: "5.0E-01",
+                   53: "1.0E-02", 55: "6.0E-01"}, 35),
+    "loss2s": _opt(dict(IZCON=1, ITCON=2, **_MAGP),
+                   {**_MAG, 12: "2.0E-01", 14: "1.5E00", 15: "2.5E00", 16: "4.0E-01", 55: "8.0E-01"}, 36,
+                   ampb="2.0D-02"),
+    # shear boundaries (no T advection, no viscous heating)
+    "shear": _opt(dict(IZCON=1, ITCON=0, LSHR=".TRUE."), _HYD, 37),
+    # density diffusion 1/rho-weighted and boundary-confined
+    # (the horizontal means these branches take need the uniform grid, NGRID=0: sub:3161-3173)
+    "ddiff1": _opt(dict(IZCON=1, ITCON=0, ID=1, DH="1.0E-02", DV="2.0E-02", NGRID=0, **_MAGP), _MAG, 38,
+                   ampb="2.0D-02"),
+    "ddiff2": _opt(dict(IZCON=0, ITCON=1, ID=2, DH="5.0E-03", DV="1.0E-02", NGRID=0), _HYD, 39),
+    # Rempel conduction of T perturbations with the radiative background
+    "rem": _opt(dict(IZCON=1, ITCON=1, LREM=".TRUE.", NGRID=0), _HYD, 40, kappa="1.0D0+0.5D0*Z*RHO"),
+    # flux relaxation (fixed T top, fixed flux bottom)
+    "rlax": dict(params=dict(NPX=12, NPY=8, NPZ=12, NPEY=2, NPEZ=3, IZCON=1, ITCON=0, NGRID=0),
+                 par={**_HYD, 58: "0.0E00", 59: "5.0E-02", 60: "5.0E-01"}, steps=3, dumps=[1, 3],
+                 ampu="2.0D-02", ampb="0.0D0", seed=41),
+}
+CONFIGS.update(OPT_CONFIGS)
+
 # Runs of the reference's own main program (3dmhd.f) with its output files
 # (.dat0 per rank, .par, .lis): fresh start from STATIC, a dump after every
 # step.  The drop-in STEP and the host driver must write the same bytes.

@@ -56,6 +56,7 @@ C
       COMMON/RUNGKU/GAM1,GAM2,GAM3,ZETA1,ZETA2
       COMMON/COMMUN/MYPE,MYPEY,MYPEZ,MPISIZE
       COMMON/RELAX/TSTART,TOFF,RLAX
+      COMMON/SPECIALBOUND/TU,DZTB,DZTU
 C
       CALL MPI_INIT(IERR)
       CALL MPI_COMM_RANK(MPI_COMM_WORLD,MYPE,IERR)
@@ -156,6 +157,20 @@ C     the local metric arrays.
      4   OMX,OMZ,ORM,RM,OBETA,TB,TP,DZF,AMPT
       WRITE(21) EXX,DXXDX,D2XXDX2,DDX,WYY,DYYDY,D2YYDY2,DDY,
      2   ZEE,DZZDZ,D2ZZDZ2,DDZ,RKAPA,DKAPA
+      CLOSE(21)
+C     Parameters of the optional FLUXES/BCON branches (density diffusion,
+C     Rempel conduction, shear, relaxation, plume and heat loss).
+      WRITE(FNAME,'(A,I4.4,A)') 'opt_r',MYPE,'.bin'
+      OPEN(21,FILE=FNAME,ACCESS='STREAM',FORM='UNFORMATTED',
+     2     STATUS='REPLACE')
+      WRITE(21) DBLE(ID),DH,DV,TU,DZTU,DZTB,RLAX,TSTART,TOFF,
+     2   XP,YP,ZP,TC,QFH,HH,XMAX,YMAX,ZMAX
+      LOPT=0
+      IF (LREM) LOPT=1
+      WRITE(21) DBLE(LOPT)
+      LOPT=0
+      IF (LSHR) LOPT=1
+      WRITE(21) DBLE(LOPT)
       CLOSE(21)
 C
       WRITE(FNAME,'(A,I4.4,A)') 'init_r',MYPE,'.bin'

@@ -6,6 +6,7 @@ recipe in oracle/build_ref.py, runs the harness (oracle/refharness.f) under
 MPI, stitches the per-rank dumps into single-domain padded arrays and saves
 
   <name>.npz   scalars, global metric arrays, bare polytrope profiles, and
+               the parameters of the optional FLUXES/BCON branches, and
                (small boxes) full initial state, one FLUXES evaluation and the
                states after the first and last dumped steps; every dumped
                step also gets a SHA-256 of its full padded state and its
@@ -45,6 +46,8 @@ CONST_KEYS = ["hx", "h2x", "hy", "h2y", "hz", "h2z", "c13", "c23", "c43",
               "gam1", "gam2", "gam3", "zeta1", "zeta2",
               "cv", "ocv", "ore", "re", "repr", "theta", "grav", "sf", "gamma",
               "omx", "omz", "orm", "rm", "obeta", "tb", "tp", "dzf", "ampt"]
+OPT_KEYS = ["id", "dh", "dv", "tu", "dztu", "dztb", "rlax", "tstart", "toff",
+            "xp", "yp", "zp", "tc", "qfh", "hh", "xmax", "ymax", "zmax", "lrem", "lshr"]
 # Boxes above this many padded points keep checksums only.
 FULL_LIMIT = 40_000
 
@@ -126,6 +129,13 @@ def stitch(run_dir: str, cfg: dict) -> dict:
                 dst[k0:k0 + v.size] = v
     out.update(mx)
     out["prof_t"], out["prof_r"], out["prof_bx"] = prof_t, prof_r, prof_bx
+    # parameters of the optional branches (the same on every rank)
+    opt = os.path.join(run_dir, "opt_r0000.bin")
+    if os.path.exists(opt):
+        v = np.fromfile(opt, dtype=np.float64)
+        assert v.size == len(OPT_KEYS)
+        for k, x in zip(OPT_KEYS, v):
+            out[k] = int(x) if k in ("id", "lrem", "lshr") else float(x)
 
     def fields_from(prefix, names, nextra=0):
         glob = {k: np.zeros((gx, gy, gz), order="F") for k in names}
