@@ -45,7 +45,7 @@ enum {
 typedef struct mhd_desc {
     int npx, npy, npz;          /* interior points; z includes both boundary planes */
     int lmag, lrot;             /* LMAG, LROT */
-    int izcon, itcon, ibcon;    /* bottom T (0 fixed, 1 flux), top T (0/1), B (0: BZ=0) */
+    int izcon, itcon, ibcon;    /* bottom T (0 fixed, 1 flux), top T (0 fixed, 1 flux, 2-4 below), B (0: BZ=0) */
     int precision;              /* MHD_FP64 (bit-identical to the reference) or MHD_MIXED */
     double hx, h2x, hy, h2y, hz, h2z, c13, c23, c43;   /* /GRID/ */
     double gam1, gam2, gam3, zeta1, zeta2;              /* /RUNGKU/ */
@@ -53,8 +53,21 @@ typedef struct mhd_desc {
     double omx, omz;                                    /* /CROT/ */
     double orm, rm, obeta;                              /* /CMAG/ */
     double tb;                                          /* /CPEN/ TB */
-    double tp;                                          /* /CPER/ TP (must be 0) */
+    double tp;                                          /* /CPER/ TP */
     double dzf;             /* float32 1.0E00/FLOAT(NPZ-1) widened (BCON, sub:2856/2929) */
+    /* Optional branches of FLUXES and BCON (MHD_FP64 only; all zero: the
+     * main path).  itcon may be 0..4: 2 embedded heat loss (sub:1783-1806),
+     * 3 top T left to the equations, 4 TP cooler on half the top (2828-2850);
+     * tp != 0 with itcon 0/1 is the plume boundary (2790-2827). */
+    int id;                 /* 3dmhdparam.f ID: density diffusion 0, 1 (1/ROM), 2 (boundary-confined), sub:1301-1371 */
+    int lrem, lshr;         /* LREM: Rempel conduction (sub:1536-1581); LSHR: shear boundaries (1493, 2751-2770) */
+    int npey, npez;         /* the reference decomposition whose summation order the horizontal means
+                               (HORIZONTAL_MEAN, sub:3143-3223) and the relaxation integral follow; 0 = 1 */
+    double dh, dv;          /* 3dmhdparam.f DH, DV */
+    double tu, dztu, dztb;  /* /SPECIALBOUND/ TU, DZTU, DZTB (LREM boundary values) */
+    double rlax, tstart, toff;          /* /RELAX/ flux relaxation (sub:1615-1780), fixed-T top, fixed-flux bottom */
+    double xp, yp, zp, tc, qfh, hh;     /* /CPER/ plume and heat-loss position, times and width */
+    double zmax;            /* /BOUNDS/ ZMAX (ID=2) */
 } mhd_desc;
 
 /* Precision of the step (mhd_desc.precision). */
@@ -78,6 +91,12 @@ int mhd_set_metrics(mhd_t *h,
                     const double *dyydy, const double *d2yydy2, const double *ddy,
                     const double *dzzdz, const double *d2zzdz2, const double *ddz,
                     const double *rkapa, const double *dkapa);
+
+/* The coordinates EXX, WYY, ZEE of /AJACOBI/ (3dmhd.f:58-59; lengths NX,
+ * NY, NZ).  Needed before the first step by the branches that use them: the
+ * plume boundary (tp != 0, itcon 0/1), the embedded heat loss (itcon 2),
+ * density diffusion ID=2 and the flux relaxation. */
+int mhd_set_coordinates(mhd_t *h, const double *exx, const double *wyy, const double *zee);
 
 /* MHD_MIXED only (no counterpart in the reference): the hydrostatic
  * background rho0(z), T0(z) about which perturbations are carried, padded
@@ -104,7 +123,10 @@ int mhd_download(mhd_t *h, double *ru, double *rv, double *rw, double *ro, doubl
 int mhd_step(mhd_t *h, double *dt, double *umach);
 
 /* n steps back to back with no host synchronisation in between (the
- * driver's DO 1000 loop without dumps); dt/umach of the last step. */
+ * driver's DO 1000 loop without dumps); dt/umach of the last step.  (With a
+ * time-dependent plume boundary or heat-loss ramp, TC != 0, each step first
+ * reads TIMT back: the host evaluates EXP/TANH of it as the reference's
+ * Fortran runtime does.) */
 int mhd_steps(mhd_t *h, int n, double *dt, double *umach);
 
 /* Step counter and clocks of /ITER/ NIT and /CTIM/ TIMC, TIMT, TIMI as STEP

@@ -33,8 +33,16 @@ METRICS = ("dxxdx", "d2xxdx2", "ddx", "dyydy", "d2yydy2", "ddy",
            "dzzdz", "d2zzdz2", "ddz", "rkapa", "dkapa")
 
 
+# optional FLUXES/BCON branches (mhd_desc members after dzf)
+_DESC_OPT_INTS = ["id", "lrem", "lshr", "npey", "npez"]
+_DESC_OPT_DBLS = ["dh", "dv", "tu", "dztu", "dztb", "rlax", "tstart", "toff", "xp", "yp", "zp", "tc", "qfh", "hh",
+                  "zmax"]
+COORDS = ("exx", "wyy", "zee")
+
+
 class MhdDesc(C.Structure):
-    _fields_ = [(k, C.c_int) for k in _DESC_INTS] + [(k, C.c_double) for k in _DESC_DBLS]
+    _fields_ = ([(k, C.c_int) for k in _DESC_INTS] + [(k, C.c_double) for k in _DESC_DBLS]
+                + [(k, C.c_int) for k in _DESC_OPT_INTS] + [(k, C.c_double) for k in _DESC_OPT_DBLS])
 
 
 class MhdDevMetrics(C.Structure):
@@ -65,6 +73,7 @@ def load(path: str | None = None):
     lib.mhd_create.argtypes = [C.POINTER(MhdDesc), C.POINTER(C.c_void_p)]
     lib.mhd_set_metrics.argtypes = [C.c_void_p] + [_P] * 11
     lib.mhd_set_background.argtypes = [C.c_void_p, _P, _P]
+    lib.mhd_set_coordinates.argtypes = [C.c_void_p, _P, _P, _P]
     lib.mhd_upload.argtypes = [C.c_void_p] + [_P] * 8
     lib.mhd_download.argtypes = [C.c_void_p] + [_P] * 8
     lib.mhd_step.argtypes = [C.c_void_p, _P, _P]
@@ -136,7 +145,7 @@ def load(path: str | None = None):
                "mhd_ens_get_tiling", "mhd_ens_set_tile", "mhd_ens_get_tile", "mhd_ens_autotune",
                "mhd_ens_time_steps"):
         getattr(lib, fn).restype = C.c_int
-    for fn in ("mhd_create", "mhd_set_metrics", "mhd_set_background", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
+    for fn in ("mhd_create", "mhd_set_metrics", "mhd_set_coordinates", "mhd_set_background", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
                "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps", "mhd_set_fixed_dt", "mhd_box_filter",
                "mhd_flux_profiles"):
         getattr(lib, fn).restype = C.c_int
@@ -154,8 +163,10 @@ def make_desc(meta: dict, precision: str = "fp64") -> MhdDesc:
     d.izcon = int(meta.get("izc", meta.get("izcon", 0)))
     d.itcon = int(meta.get("itc", meta.get("itcon", 0)))
     d.ibcon = int(meta.get("ibc", meta.get("ibcon", 0)))
-    for k in _DESC_DBLS:
+    for k in _DESC_DBLS + _DESC_OPT_DBLS:
         setattr(d, k, float(meta.get(k, 0.0)))
+    for k in _DESC_OPT_INTS:
+        setattr(d, k, int(meta.get(k, 0)))
     return d
 
 
@@ -204,6 +215,14 @@ class Box:
             raise MhdError(rc, self.lib.mhd_last_error(None).decode())
         self.h = h
         self._check(self.lib.mhd_set_metrics(self.h, *[_ptr(a) for a in self._metrics]))
+        if all(k in meta for k in COORDS):
+            # EXX, WYY, ZEE: used by the plume, heat-loss, ID=2 and relaxation branches
+            lens = (self.shape[0], self.shape[1], self.shape[2])
+            self._coords = [np.ascontiguousarray(meta[k], dtype=np.float64) for k in COORDS]
+            for k, a, n in zip(COORDS, self._coords, lens):
+                if a.shape != (n,):
+                    raise ValueError(f"coordinate {k}: {a.shape}, expected ({n},)")
+            self._check(self.lib.mhd_set_coordinates(self.h, *[_ptr(a) for a in self._coords]))
 
     def _check(self, rc):
         if rc:

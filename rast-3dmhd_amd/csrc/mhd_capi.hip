@@ -57,6 +57,16 @@ struct mhd_ctx {
     int cur = 0;
     bool have_metrics = false, have_state = false, have_bg = false;
     bool cfl_valid = false;             // sc->dt/umach describe the current register
+    // optional FLUXES/BCON branches (FP64): per-plane tables, relaxation state,
+    // the T right-hand side carried between evaluations (LSHR)
+    double *optmem = nullptr;
+    RelaxState *rs = nullptr;
+    double *ftacc = nullptr;
+    OptBuffers ob{};
+    bool need_coords = false, have_coords = false;
+    bool means = false;                 // plane means before every evaluation
+    bool timed = false;                 // EXP/TANH of TIMT per step (TC != 0)
+    mhd_desc d{};
     char err[512] = {0};
 };
 
@@ -93,11 +103,50 @@ static void free_all(mhd_t *h) {
     if (!h) return;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *bufs[18] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->rr, h->bgd, h->bgf, h->bgp,
-                      h->f, h->stage, h->met, h->sc, h->force, h->force_on, h->cx, h->nrm};
+    void *bufs[21] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->rr, h->bgd, h->bgf, h->bgp,
+                      h->f, h->stage, h->met, h->sc, h->force, h->force_on, h->cx, h->nrm,
+                      h->optmem, h->rs, h->ftacc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+// Device buffers of the optional branches: the per-plane and per-axis tables
+// (ROM, WWZ, TTM, HRAD, radd, the Gaussians GX, GY, GZ, ZEE, the CFL
+// conductivity), the slab sums and relaxation work space, the relaxation
+// state (DATA values, sub:1241-1244) and, with LSHR, the carried FT.
+static hipError_t create_branches(mhd_t *h) {
+    Params &p = h->p;
+    const size_t npl = (size_t)p.npz + 2;
+    const size_t n = 8 * npl + (p.npx + 2) + (p.npy + 2) + opt_part_len(p) + opt_work_len(p);
+    hipError_t e = hipMalloc(&h->optmem, n * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(h->optmem, 0, n * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&h->rs, sizeof(RelaxState));
+    if (e == hipSuccess) {
+        const RelaxState r0{1.0, -1e99, 1e99, 0};
+        e = hipMemcpy(h->rs, &r0, sizeof(r0), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && p.lshr) {
+        e = hipMalloc(&h->ftacc, (size_t)p.fsize * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(h->ftacc, 0, (size_t)p.fsize * sizeof(double));  // FT starts at zero
+    }
+    if (e != hipSuccess) return e;
+    double *b = h->optmem;
+    OptBuffers &o = h->ob;
+    o.rom = b; o.wwz = b + npl; o.ttm = b + 2 * npl; o.hrad = b + 3 * npl; o.radd = b + 4 * npl;
+    double *gz = b + 5 * npl, *zee = b + 6 * npl;
+    double *gx = b + 8 * npl, *gy = gx + (p.npx + 2);
+    o.part = gy + (p.npy + 2);
+    o.work = o.part + opt_part_len(p);
+    o.rs = h->rs;
+    o.zee = zee;
+    p.rom = o.rom; p.wwz = o.wwz; p.ttm = o.ttm; p.hrad = o.hrad; p.radd = o.radd;
+    p.gx = gx; p.gy = gy; p.gz = gz;
+    p.ftacc = h->ftacc;
+    h->need_coords = desc_needs_coords(&h->d);
+    h->means = p.id != 0 || p.lrem || p.rlax > 0.0;
+    h->timed = p.tc != 0.0 && ((p.itc == 2) || (p.tp != 0.0 && (p.itc == 0 || p.itc == 1)));
+    return hipSuccess;
 }
 
 extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
@@ -111,6 +160,7 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
     mhd_t *h = new (std::nothrow) mhd_ctx();
     if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_create: out of host memory");
     h->desc = *d;
+    h->d = *d;
     h->mixed = d->precision == MHD_MIXED;
     h->p = params_from_desc(d);
     const Params &p = h->p;
@@ -148,6 +198,13 @@ extern "C" int mhd_create(const mhd_desc *d, mhd_t **out) {
         delete h;
         return rc;
     }
+    if (e == hipSuccess && desc_branches(d)) e = create_branches(h);
+    if (e != hipSuccess) {
+        rc = fail(nullptr, MHD_ERR_HIP, "mhd_create: device allocation failed: %s", hipGetErrorString(e));
+        free_all(h);
+        delete h;
+        return rc;
+    }
     const DevScalars s0 = initial_scalars();
     e = hipMemcpy(h->sc, &s0, sizeof(s0), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -176,6 +233,15 @@ extern "C" int mhd_set_metrics(mhd_t *h, const double *dxxdx, const double *d2xx
         off += len[n];
     }
     h->m = mt;
+    if (h->p.lrem) {
+        // CFL conductivity 1+RKAPA (sub:844-852), in the slot after ZEE
+        std::vector<double> rk1((size_t)h->p.npz + 2);
+        for (size_t k = 0; k < rk1.size(); ++k) rk1[k] = 1.0 + rkapa[k];
+        double *dst = h->optmem + 7 * ((size_t)h->p.npz + 2);
+        HIPCHK(h, hipMemcpyAsync(dst, rk1.data(), rk1.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        h->m.rkcfl = dst;
+    }
     // the grid-only CFL term, which the stage kernels take from here
     const double d2 = dd2_min(h->p, ddx, ddy, ddz);
     HIPCHK(h, hipMemcpyAsync(reinterpret_cast<char *>(h->sc) + offsetof(DevScalars, dd2min), &d2, sizeof(d2),
@@ -189,6 +255,59 @@ extern "C" int mhd_set_metrics(mhd_t *h, const double *dxxdx, const double *d2xx
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->have_metrics = true;
     return MHD_OK;
+}
+
+extern "C" int mhd_set_coordinates(mhd_t *h, const double *exx, const double *wyy, const double *zee) {
+    if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_set_coordinates: null handle");
+    if (!exx || !wyy || !zee) return fail(h, MHD_ERR_ARG, "mhd_set_coordinates: null array");
+    if (!h->optmem) return MHD_OK;  // no branch reads them
+    const Params &p = h->p;
+    const mhd_desc &d = h->d;
+    const size_t nx = p.npx + 2, ny = p.npy + 2, nz = (size_t)p.npz + 2;
+    // CLN = -4.0E00*LOG(2.0E00)/HH/HH: the float32 product, then double
+    // divisions (sub:1785, 2803); EXP(CLN*(EXX(I)-XP)**2) per index (the host C
+    // library's exp, as the reference's Fortran runtime)
+    const double cln = -((double)(4.0f * logf(2.0f)) / d.hh / d.hh);
+    std::vector<double> gx(nx), gy(ny), gz(nz), wwz(nz);
+    for (size_t i = 0; i < nx; ++i) gx[i] = exp(cln * ((exx[i] - d.xp) * (exx[i] - d.xp)));
+    for (size_t j = 0; j < ny; ++j) gy[j] = exp(cln * ((wyy[j] - d.yp) * (wyy[j] - d.yp)));
+    for (size_t k = 0; k < nz; ++k) gz[k] = exp(cln * ((zee[k] - d.zp) * (zee[k] - d.zp)));
+    HIPCHK(h, hipSetDevice(h->dev));
+    if (d.id == 2) {
+        // WWZ = EXP(CLN*ZEE**2)+EXP(CLN*(ZEE-ZMAX)**2), SGM=0.1E00 (sub:1327-1331)
+        const double sgm = (double)0.1f;
+        const double c2 = -((double)(4.0f * logf(2.0f)) / sgm / sgm);
+        for (size_t k = 0; k < nz; ++k) {
+            const double zm = zee[k] - d.zmax;
+            wwz[k] = exp(c2 * (zee[k] * zee[k])) + exp(c2 * (zm * zm));
+        }
+        HIPCHK(h, hipMemcpyAsync(h->ob.wwz, wwz.data(), nz * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    HIPCHK(h, hipMemcpyAsync(const_cast<double *>(p.gx), gx.data(), nx * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(const_cast<double *>(p.gy), gy.data(), ny * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(const_cast<double *>(p.gz), gz.data(), nz * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(const_cast<double *>(h->ob.zee), zee, nz * sizeof(double), hipMemcpyHostToDevice,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->have_coords = true;
+    return MHD_OK;
+}
+
+// The factors of the time-dependent branches at this step's TIMT (the
+// reference evaluates them with its Fortran runtime's TANH and EXP; the host
+// C library here): the heat-loss ramp 1+TANH(LOG(3.0E00)/QFH*(TIMT-TC))
+// (sub:1791) and the plume TPR = TP+(1-TP)*EXP(-TIMT/TC) (sub:2794-2796).
+static hipError_t time_factors(mhd_t *h) {
+    if (!h->timed) return hipSuccess;
+    double timt = 0.0;
+    hipError_t e = hipMemcpyAsync(&timt, reinterpret_cast<const char *>(h->sc) + offsetof(DevScalars, timt),
+                                  sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return e;
+    const mhd_desc &d = h->d;
+    h->p.ramp = 1.0 + tanh((double)logf(3.0f) / d.qfh * (timt - d.tc));
+    h->p.tpr = d.tp + (1.0 - d.tp) * exp(-(timt / d.tc));
+    return hipSuccess;
 }
 
 extern "C" int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0) {
@@ -272,7 +391,7 @@ extern "C" int mhd_download(mhd_t *h, double *ru, double *rv, double *rw, double
 // stand-alone reduction.
 static hipError_t launches(mhd_t *h, hipEvent_t *ev) {
     const Params &p = h->p;
-    hipError_t e = hipSuccess;
+    hipError_t e = time_factors(h);
     auto rec = [&](int i) {
         if (ev && e == hipSuccess) e = hipEventRecord(ev[i], h->stream);
     };
@@ -293,6 +412,8 @@ static hipError_t launches(mhd_t *h, hipEvent_t *ev) {
     int a = h->cur, b = 1 - h->cur;
     for (int st = 1; st <= 3 && e == hipSuccess; ++st) {
         rec(2 + 4 * (st - 1));
+        if (h->means) e = launch_opt_means(p, h->m, h->q[a], h->ob, h->sc, h->stream);
+        if (e != hipSuccess) break;
         if (!h->mixed) e = launch_stage(st, p, h->m, h->q[a], h->q[b], h->z, nullptr, h->sc, h->stream);
         else e = launch_stage_mixed(st, p, h->m, h->bg, h->hq[a], h->hq[b], h->dr, h->rr, h->sc, h->stream);
         rec(3 + 4 * (st - 1));
@@ -330,7 +451,19 @@ static int ready(mhd_t *h, const char *who) {
     if (!h) return fail(nullptr, MHD_ERR_ARG, "%s: null handle", who);
     if (!h->have_metrics) return fail(h, MHD_ERR_STATE, "%s: metrics not set", who);
     if (!h->have_state) return fail(h, MHD_ERR_STATE, "%s: no state uploaded", who);
+    if (h->need_coords && !h->have_coords)
+        return fail(h, MHD_ERR_STATE, "%s: the branches of this box need mhd_set_coordinates", who);
     return MHD_OK;
+}
+
+// One FLUXES evaluation of the FP64 form into h->f (plane means first).
+static hipError_t rhs_eval(mhd_t *h) {
+    hipError_t e = hipSuccess;
+    if (!h->f) e = hipMalloc(&h->f, (size_t)h->p.fsize * sizeof(double) * nfields(h));
+    if (e == hipSuccess) e = time_factors(h);
+    if (e == hipSuccess && h->means) e = launch_opt_means(h->p, h->m, h->q[h->cur], h->ob, h->sc, h->stream);
+    if (e == hipSuccess) e = launch_stage(0, h->p, h->m, h->q[h->cur], nullptr, nullptr, h->f, h->sc, h->stream);
+    return e;
 }
 
 extern "C" int mhd_step(mhd_t *h, double *dt, double *umach) {
@@ -443,8 +576,7 @@ extern "C" int mhd_fluxes(mhd_t *h, double *fu, double *fv, double *fw, double *
     if (h->mixed) return fail(h, MHD_ERR_SCOPE, "mhd_fluxes: only for MHD_FP64 handles");
     HIPCHK(h, hipSetDevice(h->dev));
     const int nf = nfields(h);
-    if (!h->f) HIPCHK(h, hipMalloc(&h->f, (size_t)h->p.fsize * sizeof(double) * nf));
-    HIPCHK(h, launch_stage(0, h->p, h->m, h->q[h->cur], nullptr, nullptr, h->f, h->sc, h->stream));
+    HIPCHK(h, rhs_eval(h));
     double *dst[8] = {fu, fv, fw, fr, ft, ww1, ww2, ww3};
     const size_t pbytes = (size_t)(h->p.npx + 2) * (h->p.npy + 2) * (h->p.npz + 2) * sizeof(double);
     for (int f = 0; f < nf; ++f) {
@@ -526,8 +658,7 @@ extern "C" int mhd_box_filter(mhd_t *h, int factor, int zmode, int source, doubl
     HIPCHK(h, hipSetDevice(h->dev));
     const double *src = h->q[h->cur];
     if (source == 1) {
-        if (!h->f) HIPCHK(h, hipMalloc(&h->f, (size_t)p.fsize * sizeof(double) * nf));
-        HIPCHK(h, launch_stage(0, p, h->m, h->q[h->cur], nullptr, nullptr, h->f, h->sc, h->stream));
+        HIPCHK(h, rhs_eval(h));
         src = h->f;
     }
     const size_t bytes = (size_t)nf * nzc * nyc * nxc * sizeof(double);

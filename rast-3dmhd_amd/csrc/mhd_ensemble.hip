@@ -157,6 +157,8 @@ extern "C" int mhd_ens_create(const mhd_desc *descs, int nbox, mhd_ens_t **out) 
         char msg[400];
         const int rc = check_desc(&descs[b], msg, sizeof msg);
         if (rc) return efail(nullptr, rc, "mhd_ens_create: box %d: %s", b, msg);
+        if (desc_branches(&descs[b]))
+            return efail(nullptr, MHD_ERR_SCOPE, "mhd_ens_create: box %d: the ensemble has the main path only", b);
         if ((descs[b].lmag != 0) != (descs[0].lmag != 0) || descs[b].precision != descs[0].precision)
             return efail(nullptr, MHD_ERR_ARG, "mhd_ens_create: box %d: lmag and precision must match box 0", b);
     }

@@ -15,6 +15,16 @@ namespace rmhd {
 // handle) and returns code (mhd_capi.hip).
 int global_fail(int code, const char *fmt, ...);
 
+// Whether a descriptor switches on any optional branch of FLUXES/BCON.
+inline bool desc_branches(const mhd_desc *d) {
+    return d->id != 0 || d->lrem || d->lshr || d->rlax > 0.0 || d->itcon > 1 || d->tp != 0.0;
+}
+
+// Branches whose kernels read the coordinate tables (EXX, WYY, ZEE).
+inline bool desc_needs_coords(const mhd_desc *d) {
+    return (d->tp != 0.0 && (d->itcon == 0 || d->itcon == 1)) || d->itcon == 2 || d->id == 2 || d->rlax > 0.0;
+}
+
 // Checks a descriptor; on failure writes a message to msg and returns the
 // MHD_ERR_* code.
 inline int check_desc(const mhd_desc *d, char *msg, size_t len) {
@@ -26,17 +36,38 @@ inline int check_desc(const mhd_desc *d, char *msg, size_t len) {
         snprintf(msg, len, "IZCON=%d (BCON: Invalid IZCON)", d->izcon);
         return MHD_ERR_SCOPE;
     }
-    if (d->itcon != 0 && d->itcon != 1) {
-        snprintf(msg, len, "ITCON=%d not supported (0 or 1)", d->itcon);
-        return MHD_ERR_SCOPE;
-    }
-    if (d->tp != 0.0) {
-        snprintf(msg, len, "plume boundary (TP != 0) not supported");
+    if (d->itcon < 0 || d->itcon > 4) {
+        snprintf(msg, len, "ITCON=%d not supported (0..4)", d->itcon);
         return MHD_ERR_SCOPE;
     }
     if (d->precision != MHD_FP64 && d->precision != MHD_MIXED) {
         snprintf(msg, len, "precision %d unknown", d->precision);
         return MHD_ERR_ARG;
+    }
+    if (d->id < 0 || d->id > 2) {
+        snprintf(msg, len, "ID=%d not supported (0..2)", d->id);
+        return MHD_ERR_SCOPE;
+    }
+    if (d->rlax > 0.0 && !(d->itcon == 0 && d->izcon == 1)) {
+        // the reference stops for other pairs (sub:1642-1646), and its fixed-flux
+        // top / fixed-T bottom integral reads FRADM(NZ), which it never sets on
+        // the bottom rank (sub:1679-1691, 1745-1748)
+        snprintf(msg, len, "flux relaxation (RLAX > 0) needs ITCON=0 and IZCON=1");
+        return MHD_ERR_SCOPE;
+    }
+    if (d->lshr && (d->lrem || d->rlax > 0.0)) {
+        snprintf(msg, len, "LSHR with LREM or RLAX (the reference uses a stale TTM there)");
+        return MHD_ERR_SCOPE;
+    }
+    const int npey = d->npey > 0 ? d->npey : 1, npez = d->npez > 0 ? d->npez : 1;
+    if (d->npy % npey || d->npz % npez) {
+        snprintf(msg, len, "NPEY=%d / NPEZ=%d do not divide NPY=%d / NPZ=%d", npey, npez, d->npy, d->npz);
+        return MHD_ERR_ARG;
+    }
+    if (desc_branches(d) && d->precision != MHD_FP64) {
+        snprintf(msg, len, "the optional FLUXES/BCON branches (ID, LREM, LSHR, RLAX, ITCON 2..4, TP) are built "
+                           "in the MHD_FP64 form only");
+        return MHD_ERR_SCOPE;
     }
     return MHD_OK;
 }
@@ -65,6 +96,13 @@ inline Params params_from_desc(const mhd_desc *d) {
     p.t_visc43 = d->c43 * d->ocv * d->ore; // sub:1962
     p.t_cross = d->c13 * d->ore;           // sub:1930
     p.t_ohm = d->obeta * d->orm * d->ocv;  // sub:2287
+    p.id = d->id; p.lrem = d->lrem ? 1 : 0; p.lshr = d->lshr ? 1 : 0;
+    p.mirror = (d->id != 0 || d->lrem) ? 1 : 0;
+    p.npey = d->npey > 0 ? d->npey : 1; p.npez = d->npez > 0 ? d->npez : 1;
+    p.dh = d->dh; p.dv = d->dv; p.tu = d->tu; p.dztu = d->dztu; p.dztb = d->dztb;
+    p.rlax = d->rlax; p.tstart = d->tstart; p.toff = d->toff; p.tp = d->tp; p.tc = d->tc; p.hh = d->hh;
+    p.tpr = d->tp;
+    p.opt = (p.id != 0 || p.lrem || p.lshr || p.rlax > 0.0 || p.itc == 2) ? 1 : 0;
     return p;
 }
 
@@ -116,6 +154,7 @@ inline Metrics metric_table(const Params &p, const double *base, size_t len[11])
     m.dyydy = a[3]; m.d2yydy2 = a[4]; m.ddy = a[5];
     m.dzzdz = a[6]; m.d2zzdz2 = a[7]; m.ddz = a[8];
     m.rkapa = a[9]; m.dkapa = a[10];
+    m.rkcfl = a[9];
     return m;
 }
 

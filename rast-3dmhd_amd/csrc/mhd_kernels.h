@@ -30,6 +30,17 @@ struct Params {
     double t_visc43;            // C43*OCV*ORE (sub:1962)
     double t_cross;             // C13*ORE (sub:1930)
     double t_ohm;               // OBETA*ORM*OCV (sub:2287)
+    // Optional branches of FLUXES/BCON (FP64 form; all zero: the main path).
+    int opt;                    // any branch below that the stage kernel must see
+    int id, lrem, lshr, mirror; // mirror: ID or LREM (ghost RO planes mirrored, sub:1301-1310)
+    int npey, npez;             // decomposition whose summation order the means follow
+    double dh, dv, tu, dztu, dztb, rlax, tstart, toff, tp, tc, hh;
+    double ramp;                // 1 + TANH(LOG(3.0E00)/QFH*(TIMT-TC)) of this step (ITC=2, TC != 0)
+    double tpr;                 // plume TPR of this step (TP with ITC=0/1)
+    // per-plane (npz+2) and per-axis device tables of the branches
+    const double *rom, *wwz, *ttm, *hrad, *radd;
+    const double *gx, *gy, *gz; // EXP(CLN*(EXX(I)-XP)**2) etc. per padded index
+    double *ftacc;              // LSHR: FT carried between evaluations (one field)
 };
 
 // Device metric table: padded arrays, same indexing as the reference
@@ -39,6 +50,7 @@ struct Metrics {
     const double *dyydy, *d2yydy2, *ddy;
     const double *dzzdz, *d2zzdz2, *ddz;
     const double *rkapa, *dkapa;
+    const double *rkcfl;        // conductivity of the CFL limit: RKAPA, or 1+RKAPA with LREM (sub:844-852)
 };
 
 // Per-step device scalars.  The reductions keep order-preserving integer
@@ -135,6 +147,31 @@ __host__ __device__ inline double dunkey(unsigned long long k) {
     unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
     return __builtin_bit_cast(double, b);
 }
+
+// Saved state of the flux relaxation (FLUXES' DATA/SAVE, sub:1241-1245).
+struct RelaxState {
+    double tfac, ttm_max, ttm_min;
+    int icall;
+};
+
+// Device buffers of the optional branches: the per-plane tables the stage
+// kernel reads (npz+2 each), slab partial sums (2*(npz+2)*npey), relaxation
+// work space (3*(npz+2)+npez), the relaxation state and the coordinates ZEE.
+struct OptBuffers {
+    double *rom, *wwz, *ttm, *hrad, *radd;
+    double *part, *work;
+    RelaxState *rs;
+    const double *zee;
+};
+inline size_t opt_part_len(const Params &p) { return 2 * (size_t)(p.npz + 2) * p.npey; }
+inline size_t opt_work_len(const Params &p) { return 3 * (size_t)(p.npz + 2) + p.npez; }
+
+// Before every FLUXES evaluation of the optional branches (mhd_opt.hip), from
+// the stage input q: the plane means ROM and TTM (HORIZONTAL_MEAN in the
+// summation order of npey slabs), WWZ of ID=1, HRAD of LREM, and the
+// relaxation increment radd = RLAX*TFAC*CORRECT with its saved state.
+hipError_t launch_opt_means(const Params &p, const Metrics &m, const double *q, const OptBuffers &o,
+                            DevScalars *sc, hipStream_t s);
 
 // Launchers (mhd_kernels.hip).
 hipError_t launch_stage(int mode, const Params &p, const Metrics &m, const double *qin, double *qout,

@@ -79,6 +79,8 @@ extern "C" int mhd_step_noalloc(const mhd_desc *d, const mhd_dev_metrics *met, d
         if (rc) return global_fail(rc, "mhd_step_noalloc: %s", msg);
     }
     if (d->precision != MHD_FP64) return global_fail(MHD_ERR_SCOPE, "mhd_step_noalloc: MHD_FP64 only");
+    if (desc_branches(d))
+        return global_fail(MHD_ERR_SCOPE, "mhd_step_noalloc: the main path only (no optional FLUXES/BCON branch)");
     const Params p = params_from_desc(d);
     const int nf = p.lmag ? 8 : 5;
     const size_t reg = (size_t)nf * (size_t)p.fsize;  // doubles per register
@@ -111,6 +113,7 @@ extern "C" int mhd_step_noalloc(const mhd_desc *d, const mhd_dev_metrics *met, d
     m.dyydy = met->dyydy; m.d2yydy2 = met->d2yydy2; m.ddy = met->ddy;
     m.dzzdz = met->dzzdz; m.d2zzdz2 = met->d2zzdz2; m.ddz = met->ddz;
     m.rkapa = met->rkapa; m.dkapa = met->dkapa;
+    m.rkcfl = met->rkapa;
 
     // scalars: keys armed, the caller's clock
     DevScalars s0 = initial_scalars();

@@ -21,6 +21,9 @@ module rast3dmhd_c
     real(c_double) :: tb
     real(c_double) :: tp
     real(c_double) :: dzf
+    integer(c_int) :: id, lrem, lshr, npey, npez
+    real(c_double) :: dh, dv, tu, dztu, dztb, rlax, tstart, toff
+    real(c_double) :: xp, yp, zp, tc, qfh, hh, zmax
   end type mhd_desc
 
   interface
@@ -37,6 +40,12 @@ module rast3dmhd_c
       type(c_ptr), value :: h
       real(c_double), intent(in) :: dxxdx(*), d2xxdx2(*), ddx(*), dyydy(*), d2yydy2(*), ddy(*)
       real(c_double), intent(in) :: dzzdz(*), d2zzdz2(*), ddz(*), rkapa(*), dkapa(*)
+    end function
+
+    integer(c_int) function mhd_set_coordinates(h, exx, wyy, zee) bind(c, name='mhd_set_coordinates')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: exx(*), wyy(*), zee(*)
     end function
 
     integer(c_int) function mhd_set_background(h, ro0, tt0) bind(c, name='mhd_set_background')

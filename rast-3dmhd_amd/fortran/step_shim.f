@@ -29,10 +29,11 @@ C     back with its ghost layers as COMMUNICATE leaves them (neighbour
 C     rows and planes, periodic wrap; the untouched physical z ghosts).
 C     With NPE=1 no MPI call is made and /BIG/ is used directly.
 C
-C     Branches of FLUXES/BCON that the GPU step does not have (ID, LREM,
-C     LSHR, RLAX; ITCON outside 0/1 and TP are refused by mhd_create)
-C     stop the run before the first step.  Errors print the library's
-C     message and stop, as the reference does for invalid settings.
+C     The optional branches of FLUXES/BCON (ID, LREM, LSHR, RLAX, ITCON
+C     2..4, TP) are passed on with the coordinates EXX, WYY, ZEE; the
+C     library refuses those it does not have in the chosen form.  Errors
+C     print the library's message and stop, as the reference does for
+C     invalid settings.
 C=======================================================================
       SUBROUTINE STEP
       USE ISO_C_BINDING
@@ -81,6 +82,8 @@ C
       COMMON/RUNGKU/GAM1,GAM2,GAM3,ZETA1,ZETA2
       COMMON/COMMUN/MYPE,MYPEY,MYPEZ,MPISIZE
       COMMON/RELAX/TSTART,TOFF,RLAX
+      COMMON/SPECIALBOUND/TU,DZTB,DZTU
+      COMMON/BOUNDS/XMAX,YMAX,ZMAX
 C
       TYPE(C_PTR), SAVE :: H
       LOGICAL, SAVE :: READY = .FALSE.
@@ -88,6 +91,8 @@ C     rank 0 of a decomposed run: the whole padded domain, /BIG/ order
 C     RU,RV,RW,RO,TT,BX,BY,BZ, and the y and z metric arrays
       REAL(C_DOUBLE), ALLOCATABLE, SAVE :: G(:,:,:,:)
       REAL(C_DOUBLE), ALLOCATABLE, SAVE :: GY(:,:), GZ(:,:)
+C     (GY: DYYDY, D2YYDY2, DDY, WYY; GZ: DZZDZ, D2ZZDZ2, DDZ, RKAPA, DKAPA,
+C     ZEE)
       REAL(C_DOUBLE), ALLOCATABLE :: BUF(:,:,:,:), MV(:), MB(:,:)
       TYPE(MHD_DESC) :: D
       INTEGER(C_INT) :: IRC
@@ -104,11 +109,6 @@ C
             WRITE(6,*)'FLUXES: Non-periodic horizontal boundaries'
             CALL SHIM_STOP
          ENDIF
-         IF ((ID.NE.0).OR.LREM.OR.LSHR.OR.(RLAX.GT.0.0E00)) THEN
-            WRITE(6,*)'STEP: the GPU step has no ID, LREM, LSHR or ',
-     2                'RLAX branch'
-            CALL SHIM_STOP
-         ENDIF
          NF=5
          IF (LMAG) NF=8
          TBG(1)=TB
@@ -118,7 +118,7 @@ C           TB is the bottom ranks' value on a restart (3dmhd.f:354-356)
      2                     MPI_COMM_WORLD,IERR)
             IF (MYPE.EQ.0) THEN
                ALLOCATE(G(NX,NPY+2,NPZ+2,NF),BUF(NX,NY,NZ,0:NPE-1))
-               ALLOCATE(GY(NPY+2,3),GZ(NPZ+2,5))
+               ALLOCATE(GY(NPY+2,4),GZ(NPZ+2,6))
             ELSE
                ALLOCATE(BUF(1,1,1,0:0))
             ENDIF
@@ -263,10 +263,11 @@ C     Rank 0 to every rank: its window of G(:,:,:,F), ghosts included.
      2                 MPI_DOUBLE_PRECISION,0,MPI_COMM_WORLD,IERR)
       END SUBROUTINE SCATTER3
 C
-C     Rank 0: the y metrics (DYYDY, D2YYDY2, DDY) and z metrics (DZZDZ,
-C     D2ZZDZ2, DDZ, RKAPA, DKAPA) of the whole domain from the slabs.
+C     Rank 0: the y metrics (DYYDY, D2YYDY2, DDY, WYY) and z metrics
+C     (DZZDZ, D2ZZDZ2, DDZ, RKAPA, DKAPA, ZEE) of the whole domain from the
+C     slabs.
       SUBROUTINE GATHER_METRICS
-      NMV=3*NY+5*NZ
+      NMV=4*NY+6*NZ
       ALLOCATE(MV(NMV))
       IF (MYPE.EQ.0) THEN
          ALLOCATE(MB(NMV,0:NPE-1))
@@ -276,11 +277,13 @@ C     D2ZZDZ2, DDZ, RKAPA, DKAPA) of the whole domain from the slabs.
       MV(1:NY)=DYYDY
       MV(NY+1:2*NY)=D2YYDY2
       MV(2*NY+1:3*NY)=DDY
-      MV(3*NY+1:3*NY+NZ)=DZZDZ
-      MV(3*NY+NZ+1:3*NY+2*NZ)=D2ZZDZ2
-      MV(3*NY+2*NZ+1:3*NY+3*NZ)=DDZ
-      MV(3*NY+3*NZ+1:3*NY+4*NZ)=RKAPA
-      MV(3*NY+4*NZ+1:3*NY+5*NZ)=DKAPA
+      MV(3*NY+1:4*NY)=WYY
+      MV(4*NY+1:4*NY+NZ)=DZZDZ
+      MV(4*NY+NZ+1:4*NY+2*NZ)=D2ZZDZ2
+      MV(4*NY+2*NZ+1:4*NY+3*NZ)=DDZ
+      MV(4*NY+3*NZ+1:4*NY+4*NZ)=RKAPA
+      MV(4*NY+4*NZ+1:4*NY+5*NZ)=DKAPA
+      MV(4*NY+5*NZ+1:4*NY+6*NZ)=ZEE
       CALL MPI_GATHER(MV,NMV,MPI_DOUBLE_PRECISION,MB,NMV,
      2                MPI_DOUBLE_PRECISION,0,MPI_COMM_WORLD,IERR)
       IF (MYPE.EQ.0) THEN
@@ -294,16 +297,18 @@ C     D2ZZDZ2, DDZ, RKAPA, DKAPA) of the whole domain from the slabs.
                GY(JG,1)=MB(J,IR)
                GY(JG,2)=MB(NY+J,IR)
                GY(JG,3)=MB(2*NY+J,IR)
+               GY(JG,4)=MB(3*NY+J,IR)
             ENDDO
             DO K=1,NZ
                IF ((K.EQ.1).AND.(PZ.GT.0)) CYCLE
                IF ((K.EQ.NZ).AND.(PZ.LT.NPEZ-1)) CYCLE
                KG=PZ*NRZ+K
-               GZ(KG,1)=MB(3*NY+K,IR)
-               GZ(KG,2)=MB(3*NY+NZ+K,IR)
-               GZ(KG,3)=MB(3*NY+2*NZ+K,IR)
-               GZ(KG,4)=MB(3*NY+3*NZ+K,IR)
-               GZ(KG,5)=MB(3*NY+4*NZ+K,IR)
+               GZ(KG,1)=MB(4*NY+K,IR)
+               GZ(KG,2)=MB(4*NY+NZ+K,IR)
+               GZ(KG,3)=MB(4*NY+2*NZ+K,IR)
+               GZ(KG,4)=MB(4*NY+3*NZ+K,IR)
+               GZ(KG,5)=MB(4*NY+4*NZ+K,IR)
+               GZ(KG,6)=MB(4*NY+5*NZ+K,IR)
             ENDDO
          ENDDO
       ENDIF
@@ -370,6 +375,30 @@ C     RAST3DMHD_PRECISION=mixed selects the mixed-precision form
       D%TB=TBG(1)
       D%TP=TP
       D%DZF=1.0E00/FLOAT(NPZ-1)
+C     optional branches (3dmhdparam.f, /SPECIALBOUND/, /RELAX/, /CPER/,
+C     /BOUNDS/) and the decomposition whose summation order they follow
+      D%ID=ID
+      D%LREM=0
+      IF (LREM) D%LREM=1
+      D%LSHR=0
+      IF (LSHR) D%LSHR=1
+      D%NPEY=NPEY
+      D%NPEZ=NPEZ
+      D%DH=DH
+      D%DV=DV
+      D%TU=TU
+      D%DZTU=DZTU
+      D%DZTB=DZTB
+      D%RLAX=RLAX
+      D%TSTART=TSTART
+      D%TOFF=TOFF
+      D%XP=XP
+      D%YP=YP
+      D%ZP=ZP
+      D%TC=TC
+      D%QFH=QFH
+      D%HH=HH
+      D%ZMAX=ZMAX
       WHAT='STEP: mhd_create'
       IRC=MHD_CREATE(D,H)
       IF (IRC.NE.0) GOTO 9
@@ -381,6 +410,13 @@ C     RAST3DMHD_PRECISION=mixed selects the mixed-precision form
          IRC=MHD_SET_METRICS(H,DXXDX,D2XXDX2,DDX,GY(:,1),GY(:,2),
      2                       GY(:,3),GZ(:,1),GZ(:,2),GZ(:,3),GZ(:,4),
      3                       GZ(:,5))
+      ENDIF
+      IF (IRC.NE.0) GOTO 9
+      WHAT='STEP: mhd_set_coordinates'
+      IF (NPE.EQ.1) THEN
+         IRC=MHD_SET_COORDINATES(H,EXX,WYY,ZEE)
+      ELSE
+         IRC=MHD_SET_COORDINATES(H,EXX,GY(:,4),GZ(:,6))
       ENDIF
       IF (IRC.NE.0) GOTO 9
       IF (D%PRECISION.EQ.MHD_MIXED) THEN

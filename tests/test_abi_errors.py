@@ -27,16 +27,21 @@ def _create(meta, **over):
     return rc, lib.mhd_last_error(None).decode(), h
 
 
-@pytest.mark.parametrize("field,value,code,text", [
-    ("izcon", 2, "MHD_ERR_SCOPE", "Invalid IZCON"),
-    ("itcon", 3, "MHD_ERR_SCOPE", "ITCON=3"),
-    ("tp", 0.5, "MHD_ERR_SCOPE", "plume"),
-    ("precision", 7, "MHD_ERR_ARG", "precision 7"),
-    ("npz", 3, "MHD_ERR_ARG", "too small"),
+@pytest.mark.parametrize("over,code,text", [
+    (dict(izcon=2), "MHD_ERR_SCOPE", "Invalid IZCON"),
+    (dict(itcon=5), "MHD_ERR_SCOPE", "ITCON=5"),
+    (dict(id=3), "MHD_ERR_SCOPE", "ID=3"),
+    (dict(rlax=0.5, itcon=1, izcon=0), "MHD_ERR_SCOPE", "RLAX > 0"),
+    (dict(lshr=1, lrem=1), "MHD_ERR_SCOPE", "LSHR with LREM"),
+    (dict(itcon=2, precision=1), "MHD_ERR_SCOPE", "MHD_FP64 form only"),
+    (dict(tp=0.5, precision=1), "MHD_ERR_SCOPE", "MHD_FP64 form only"),
+    (dict(npey=3), "MHD_ERR_ARG", "NPEY=3"),
+    (dict(precision=7), "MHD_ERR_ARG", "precision 7"),
+    (dict(npz=3), "MHD_ERR_ARG", "too small"),
 ])
-def test_create_rejects_descriptors_before_touching_the_device(field, value, code, text):
+def test_create_rejects_descriptors_before_touching_the_device(over, code, text):
     meta, _ = random_box(16, 8, 6, True, True, seed=1)
-    rc, msg, h = _create(meta, **{field: value})
+    rc, msg, h = _create(meta, **over)
     assert rc == getattr(H.pkg(), code), (rc, msg)
     assert text in msg and h.value is None
 
@@ -126,7 +131,7 @@ def test_noalloc_step_rejects_before_touching_the_device():
     rc, msg = _noalloc_call(meta, precision=pkg.MHD_MIXED)
     assert rc == pkg.MHD_ERR_SCOPE and "MHD_FP64" in msg, (rc, msg)
     rc, msg = _noalloc_call(meta, itcon=3)
-    assert rc == pkg.MHD_ERR_SCOPE and "ITCON=3" in msg, (rc, msg)
+    assert rc == pkg.MHD_ERR_SCOPE and "main path only" in msg, (rc, msg)
     rc, msg = _noalloc_call(meta)  # fits, but the metric pointers are null
     assert rc == pkg.MHD_ERR_ARG and "metric array 0 is null" in msg, (rc, msg)
     lib = pkg.load()

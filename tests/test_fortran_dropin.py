@@ -121,15 +121,16 @@ def test_shim_rejects_unknown_precision(tmp_path):
 
 
 @pytest.mark.parametrize("box,data", [("lshr16", "mhdrot16"), ("id16", "hydro16")])
-def test_fortran_dropin_refuses_missing_branches(tmp_path, box, data):
-    """A parameter block with a FLUXES/BCON branch the GPU step does not
-    have (LSHR, ID; likewise LREM and RLAX > 0) stops before the first step
-    with a message, as the reference stops on invalid settings: no device
-    call is made, so this runs without a GPU."""
+def test_fortran_dropin_refuses_branches_in_the_mixed_form(tmp_path, box, data):
+    """The shim passes the optional branches (here LSHR, ID) to the library,
+    which has them in the FP64 form only: with RAST3DMHD_PRECISION=mixed,
+    mhd_create refuses them (MHD_ERR_SCOPE) before any device call and the
+    shim stops with the library's message, so this runs without a GPU."""
     exe = os.path.join(FBUILD, box, "rastreplay")
     assert os.path.exists(exe), "build the Fortran drop-in first (__graft_entry__.build())"
     g = H.load(data)
     write_replay_input(tmp_path / "replay.in", g, H.initial_state(data), 1)
-    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
-    assert "the GPU step has no ID, LREM, LSHR or RLAX branch" in r.stdout, r.stdout + r.stderr
+    env = dict(os.environ, RAST3DMHD_PRECISION="mixed")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
+    assert "STEP: mhd_create" in r.stdout and "MHD_FP64 form only" in r.stdout, r.stdout + r.stderr
     assert not os.path.exists(tmp_path / "replay.out")
