@@ -142,6 +142,49 @@ def ensemble(pkg, nbox=64, steps=100, precision="mixed", tiling=None):
             "ms_per_step_boxes_one_at_a_time": ms_single, "speedup_vs_one_at_a_time": ms_single / ms}
 
 
+def host_cores():
+    """Cores this process may use: the physical cores (distinct physical id,
+    core id) among the CPUs in its affinity mask, capped by the CPU share the
+    environment gives (OMP_NUM_THREADS, when set); and the CPU model."""
+    cpus = sorted(os.sched_getaffinity(0))
+    phys, model = {}, "unknown"
+    try:
+        cur = {}
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().split("\n") + [""]:
+                if not line.strip():
+                    if "processor" in cur:
+                        phys[int(cur["processor"])] = (cur.get("physical id", "0"), cur.get("core id", cur["processor"]))
+                        model = cur.get("model name", model)
+                    cur = {}
+                elif ":" in line:
+                    k, v = line.split(":", 1)
+                    cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    npc = len({phys.get(c, ("?", str(c))) for c in cpus}) if phys else len(cpus)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    cores = min(npc, share) if share > 0 else npc
+    return {"cores": max(1, cores), "physical_cores_in_affinity": npc, "cpus_in_affinity": len(cpus),
+            "cpu_share": share or None, "model": model}
+
+
+def baseline_exe(cores):
+    """The largest reference build (oracle/_ref/bench512_r<ranks>) with
+    ranks <= cores; (path, ranks) or (None, 0)."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    try:
+        import refconfigs
+    finally:
+        sys.path.pop(0)
+    best = (None, 0)
+    for r in refconfigs.BASELINE_RANKS:
+        exe = os.path.join(REPO, "oracle", "_ref", refconfigs.baseline_name(r), "3dmhd.exe")
+        if r <= cores and os.path.exists(exe) and r > best[1]:
+            best = (exe, r)
+    return best
+
+
 def cpu_baseline_reference(exe, ranks, n1=1, n2=3, timeout=900):
     """Reference MPI build timed on this host: (wall(n2) - wall(n1)) / (n2 - n1)."""
     mpiexec = shutil.which("mpiexec") or "/opt/conda/bin/mpiexec"
@@ -190,7 +233,9 @@ def main():
     ap.add_argument("--ensemble-tiling", type=int, nargs=4, metavar=("TILE_X", "TILE_Y", "KDEPTH", "GRID"),
                     help="fixed ensemble tiling instead of autotuning (for profiling one tiling)")
     ap.add_argument("--cpu-baseline", choices=["auto", "reference", "port", "none"], default="auto")
-    ap.add_argument("--ranks", type=int, default=16)
+    ap.add_argument("--ranks", type=int, default=0,
+                    help="MPI ranks of the reference CPU baseline (default: the largest build that fits the "
+                         "host's physical cores and CPU share)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus}: the whole box runs resident on one GPU; runs on more than "
@@ -222,16 +267,18 @@ def main():
 
     cpu = None
     if args.cpu_baseline != "none":
-        exe = os.path.join(REPO, "oracle", "_ref", "bench512", "3dmhd.exe")
-        use_ref = args.cpu_baseline in ("auto", "reference") and os.path.exists(exe) and shape == (512, 512, 256)
+        host = host_cores()
+        exe, ranks = baseline_exe(args.ranks or host["cores"])
+        use_ref = args.cpu_baseline in ("auto", "reference") and exe is not None and shape == (512, 512, 256)
         if use_ref:
             try:
-                s_per_step = cpu_baseline_reference(exe, args.ranks)
-                cpu = {"value": zones / s_per_step, "unit": "zone-updates/s", "cores": args.ranks,
+                s_per_step = cpu_baseline_reference(exe, ranks)
+                cpu = {"value": zones / s_per_step, "unit": "zone-updates/s", "cores": ranks,
                        "kind": "reference",
                        "sample": f"reference Fortran/MPI build (plain FP64), {args.npx}x{args.npy}x{args.npz}, "
-                                 f"{args.ranks} ranks (NPEY=8,NPEZ=2), (wall(3 steps)-wall(1 step))/2 "
-                                 f"= {s_per_step:.2f} s/step"}
+                                 f"{ranks} ranks (NPEY={ranks // 2},NPEZ=2), (wall(3 steps)-wall(1 step))/2 "
+                                 f"= {s_per_step:.2f} s/step",
+                       "host": host}
             except Exception as e:  # fall back to the port
                 log(f"reference baseline failed: {e}")
         if cpu is None and args.cpu_baseline != "reference":

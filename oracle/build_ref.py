@@ -144,17 +144,22 @@ def build_dropins() -> list:
     return [build_dropin(n) for n in refconfigs.DRIVER_CONFIGS]
 
 
-def build_baseline() -> str:
+def build_baseline() -> list:
     """Compile the reference main program for the throughput workload
-    (refconfigs.BASELINE), used as the CPU baseline of bench.py.  Skips the
-    compile when the executable is newer than this recipe."""
-    name, cfg = refconfigs.BASELINE_NAME, refconfigs.baseline_config()
-    exe = os.path.join(REF_OUT, name, "3dmhd.exe")
-    if os.path.exists(exe) and os.path.getmtime(exe) > max(
-            os.path.getmtime(__file__), os.path.getmtime(os.path.join(HERE, "refconfigs.py"))):
-        return exe
-    build(name, cfg, main_exe=True)
-    return exe
+    (refconfigs.baseline_config) once per rank count of
+    refconfigs.BASELINE_RANKS (NPEZ=2), the CPU baseline of bench.py, into
+    oracle/_ref/bench512_r<ranks>/3dmhd.exe.  Skips a compile when the
+    executable is newer than this recipe."""
+    exes = []
+    for ranks in refconfigs.BASELINE_RANKS:
+        name = refconfigs.baseline_name(ranks)
+        cfg = refconfigs.baseline_config(npey=ranks // 2, npez=2)
+        exe = os.path.join(REF_OUT, name, "3dmhd.exe")
+        if not (os.path.exists(exe) and os.path.getmtime(exe) > max(
+                os.path.getmtime(__file__), os.path.getmtime(os.path.join(HERE, "refconfigs.py")))):
+            build(name, cfg, main_exe=True)
+        exes.append(exe)
+    return exes
 
 
 def run_harness(name: str, cfg: dict | None = None) -> str:

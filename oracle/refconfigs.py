@@ -13,6 +13,13 @@ NPEY=1), and NPEZ>=2 as the reference requires (3dmhdparam.f:11).
 ANG45 = "0.7853982E00"
 
 BASELINE_NAME = "bench512"
+# MPI rank counts the CPU baseline is built for (NPEZ=2, NPEY=ranks/2); the
+# benchmark runs the largest that fits the host's cores.
+BASELINE_RANKS = (8, 16, 32, 64)
+
+
+def baseline_name(ranks: int) -> str:
+    return f"{BASELINE_NAME}_r{ranks}"
 
 
 def baseline_config(npx=512, npy=512, npz=256, npey=8, npez=2):

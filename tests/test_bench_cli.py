@@ -29,3 +29,21 @@ def test_help_lists_the_profiling_options():
 def test_ensemble_tiling_takes_four_values():
     r = run("--gpus", "2", "--ensemble-tiling", "32", "8", "16")
     assert r.returncode == 2 and "expected 4 arguments" in r.stderr
+
+
+def test_cpu_baseline_ranks_follow_the_host(monkeypatch):
+    """The reference baseline uses the largest rank count built that fits the
+    physical cores in the affinity mask and the CPU share (OMP_NUM_THREADS),
+    and reports the CPU model."""
+    sys.path.insert(0, REPO)
+    import bench
+    monkeypatch.setenv("OMP_NUM_THREADS", "16")
+    h = bench.host_cores()
+    assert 1 <= h["cores"] <= 16 and h["cores"] <= h["physical_cores_in_affinity"]
+    assert h["model"] and h["cpu_share"] == 16
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.host_cores()["cpu_share"] is None
+    exe, ranks = bench.baseline_exe(16)
+    if exe is not None:  # built where the reference sources exist
+        assert ranks == 16 and exe.endswith("bench512_r16/3dmhd.exe")
+        assert bench.baseline_exe(12)[1] == 8 and bench.baseline_exe(4) == (None, 0)
