@@ -7,7 +7,10 @@ pinned by mathematics instead of reference vectors:
     and agreement with a direct quadruple loop on tiny inputs;
   * the backward pass: central finite differences of the FP64 forward;
   * GELU: its closed form 0.5 x (1 + erf(x / sqrt 2)) and derivative;
-  * the box filter: the mean of f x f x f blocks, brute force.
+  * the box filter: the mean of f x f x f blocks, brute force;
+  * the training loss and Adam: closed forms (the MSE of a constant offset;
+    Adam's first step moves every parameter by lr * g / (|g| + eps), and a
+    constant gradient keeps that size at every step).
 
 Network: 3 layers of 3x3x3 convolutions, cin -> width -> width -> cout,
 GELU after layers 1 and 2.  Arrays are channel-last, x[z][y][x][c];
@@ -176,3 +179,33 @@ def box_filter_vertex(a, f):
     for K in range(1, nzc - 1):
         out[K] = np.tensordot(w, h[f * K - f // 2:f * K + f // 2 + 1], axes=(0, 0)) / f
     return out
+
+
+def mse_loss(y, t, cout):
+    """Training loss: mean over voxels and the first cout channels of
+    (y - t)^2, with y - t formed in float32 and squared and summed in
+    float64; and its gradient 2 (y - t) / (nvox * cout) as the kernels pass
+    it on (float32 scale times the float32 difference, rounded to bfloat16)."""
+    y = np.asarray(y, dtype=np.float32)[..., :cout]
+    t = np.asarray(t, dtype=np.float32)[..., :cout]
+    d = (y - t).astype(np.float32)
+    nvox = int(np.prod(d.shape[:-1]))
+    loss = float(np.sum(d.astype(np.float64) ** 2) / (nvox * cout))
+    gscale = np.float32(2.0 / (nvox * cout))
+    return loss, bf16((gscale * d).astype(np.float32))
+
+
+def adam_step(p, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in float32: m, v moments, bias corrections
+    c1 = 1 - b1^step, c2 = 1 - b2^step, p -= lr * (m / c1) / (sqrt(v / c2) + eps).
+    Returns the new (p, m, v)."""
+    f = np.float32
+    p, g, m, v = (np.asarray(a, dtype=np.float32) for a in (p, g, m, v))
+    b1, b2, lr, eps = f(b1), f(b2), f(lr), f(eps)
+    m = (b1 * m + (f(1) - b1) * g).astype(np.float32)
+    v = (b2 * v + (f(1) - b2) * g * g).astype(np.float32)
+    c1 = f(1) - f(np.power(b1, f(step), dtype=np.float32))
+    c2 = f(1) - f(np.power(b2, f(step), dtype=np.float32))
+    p = (p - lr * (m / c1) / (np.sqrt(v / c2) + eps)).astype(np.float32)
+    return p, m, v
+

@@ -771,9 +771,14 @@ __global__ void k_closure_forcing(const Params p, const float *__restrict__ y, c
 
 // ---------------------------------------------------------------------------
 
+// z chunks of about 32 planes (the tiles of a co-resident cohort stay closer
+// in step over a short march: 512x512x256, 8 chunks 1% faster than 4), at
+// least 4096 workgroups, at least 8 planes.
 static int chunk_m(const Params &p, int ty, int *nchunks) {
     const int ntiles = ((p.npx + MTX - 1) / MTX) * ((p.npy + ty - 1) / ty);
-    int nc = (4096 + ntiles - 1) / ntiles;
+    int nc = (p.npz + 31) / 32;
+    const int need = (4096 + ntiles - 1) / ntiles;
+    if (nc < need) nc = need;
     int maxc = p.npz / 8;
     if (maxc < 1) maxc = 1;
     if (nc > maxc) nc = maxc;

@@ -82,6 +82,36 @@ def test_training_reduces_loss():
     assert l1 < 0.25 * l0
 
 
+def test_loss_and_adam_step_match_the_oracle():
+    """The training loss (k_loss) and one Adam step (k_adam) against their
+    CPU restatement: the loss of a sample from the network's own forward
+    pass, and the parameters after one training step from the gradient the
+    backward pass gives for the loss gradient."""
+    say("case net loss and Adam")
+    nz, ny, nx = 6, 8, 16
+    rng = np.random.default_rng(21)
+    x = N.bf16(rng.standard_normal((nz, ny, nx, CIN)))
+    t = (0.3 * rng.standard_normal((nz, ny, nx, COUT))).astype(np.float32)
+    p0 = N.flatten(N.init_params(CIN, WIDTH, COUT, seed=5)).astype(np.float32)
+    with H.pkg().Net(CIN, WIDTH, COUT) as net:
+        net.set_params(p0)
+        y = net.forward(x)
+        net.add_sample(x, t)
+        loss = net.loss(0)
+        want, dy = N.mse_loss(y, t, COUT)
+        assert loss == pytest.approx(want, rel=1e-12)
+        g = net.gradient(x, dy)
+        first, _ = net.train(1, 1e-3)
+        p1 = net.params()
+    assert first == pytest.approx(want, rel=1e-12)
+    z = np.zeros_like(p0)
+    p1w, _, _ = N.adam_step(p0, g, z, z, 1, 1e-3)
+    # the same float32 formula: equal to a few float32 ulps of the parameters
+    tol = 4 * np.finfo(np.float32).eps * np.abs(p0).max()
+    assert np.abs(p1 - p1w).max() <= tol, np.abs(p1 - p1w).max()
+    assert np.abs((p0 - p1) - (p0 - p1w)).max() <= tol
+
+
 @pytest.mark.parametrize("npz,zmode", [(17, 1), (16, 0)])
 def test_box_filter_matches_oracle(npz, zmode):
     say("case box filter", npz, zmode)

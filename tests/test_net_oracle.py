@@ -121,3 +121,32 @@ def test_box_filter_vertex_brute_force():
     # a field linear in the plane index is reproduced at the coarse planes
     lin = np.broadcast_to(np.arange(f * 3 + 1, dtype=float)[:, None, None], a.shape)
     assert np.allclose(N.box_filter_vertex(lin, f)[:, 0, 0], [0, 4, 8, 12])
+
+
+def test_mse_loss_of_a_constant_offset():
+    """y = t + c on the first cout channels: loss c^2, gradient 2c/(n cout);
+    channels beyond cout do not count."""
+    rng = np.random.default_rng(1)
+    t = rng.standard_normal((3, 4, 5, 8)).astype(np.float32)
+    y = t + np.float32(0.25)
+    y[..., 7] += 100.0
+    loss, g = N.mse_loss(y, t, 7)
+    assert loss == pytest.approx(0.0625, rel=1e-6)
+    assert g.shape == (3, 4, 5, 7)
+    assert np.allclose(g, 2 * 0.25 / (60 * 7), rtol=1e-2)
+
+
+def test_adam_first_steps_closed_form():
+    """Adam's first step is lr * g / (|g| + eps) for every parameter, and a
+    constant gradient keeps the step size lr * |g| / (|g| + eps)."""
+    rng = np.random.default_rng(2)
+    p0 = rng.standard_normal(50).astype(np.float32)
+    g = rng.standard_normal(50).astype(np.float32)
+    z = np.zeros(50, np.float32)
+    p1, m, v = N.adam_step(p0, g, z, z, 1, 1e-3)
+    want = 1e-3 * g / (np.abs(g) + 1e-8)
+    tol = 4 * np.finfo(np.float32).eps * np.abs(p0).max()  # float32 rounding of p
+    assert np.allclose(p0 - p1, want, rtol=0, atol=tol)
+    p2, _, _ = N.adam_step(p1, g, m, v, 2, 1e-3)
+    assert np.allclose(p1 - p2, want, rtol=0, atol=tol)
+
