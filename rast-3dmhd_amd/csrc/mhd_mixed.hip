@@ -266,21 +266,25 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
         constexpr bool BND = decltype(bnd)::value;
         const bool pre = k < kend;
         const long long o = (long long)k * pl + ooff;
-        if (pre) {
-            load_rawf<MAG>(hin, fs, (long long)(k + 2) * pl + ooff, rp_o);
-            if (has_h) load_rawf<MAG>(hin, fs, (long long)(k + 2) * pl + hoff, rp_h);
-        }
         // D (stages 2, 3) or the packed R words (stage 1) of this point, read
         // once; R is unpacked where it is used, so that no wait for these
-        // loads comes before the plane's arithmetic
+        // loads comes before the plane's arithmetic.  Every load of the plane
+        // is issued on every path (lanes outside the box read their wrapped
+        // column, which is in range; the last plane of a chunk and the lanes
+        // without a halo point re-read points of their own), D first: the
+        // number of loads in flight is then the same on all paths, and the
+        // wait for D at the accumulation does not also wait for the prefetch.
         float dk_[8];
         unsigned rk_[4];
-        if (valid) {
 #pragma unroll
-            for (int f = 0; f < NF; ++f) {
-                if (MODE != 1) dk_[f] = __builtin_nontemporal_load(&dr[f * fs + o]);
-                else if (!(f & 1)) rk_[f >> 1] = __builtin_nontemporal_load(&rr[(f >> 1) * fs + o]);
-            }
+        for (int f = 0; f < NF; ++f) {
+            if (MODE != 1) dk_[f] = __builtin_nontemporal_load(&dr[f * fs + o]);
+            else if (!(f & 1)) rk_[f >> 1] = __builtin_nontemporal_load(&rr[(f >> 1) * fs + o]);
+        }
+        {
+            const long long kp = (long long)(pre ? k + 2 : k) * pl;
+            load_rawf<MAG>(hin, fs, kp + ooff, rp_o);
+            load_rawf<MAG>(hin, fs, kp + (has_h ? hoff : ooff), rp_h);
         }
         __syncthreads();
         if (valid) {
