@@ -13,12 +13,14 @@
 //   A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16], D[4(l/16)..+3][l%16]).  A
 //   workgroup (4 wavefronts) owns a 64 x 4 column tile and marches down a z
 //   chunk with a ring of three input planes (halo 1) in LDS; each wavefront
-//   computes one row of 64 voxels (4 M-tiles) for all output channels.  The
-//   layer's weight panel is staged in LDS once per workgroup, so the inner
-//   loop reads both operands from LDS.  The voxel stride and the weight row
-//   stride in LDS are odd numbers of 16-byte units so the 16 lanes of a
-//   fragment read hit distinct banks.  The epilogue adds the bias and
-//   applies GELU (hidden layers), or multiplies by GELU'(z) (backward).
+//   computes one row of 64 voxels (4 M-tiles) for all output channels.  Each
+//   lane holds its B fragments of the whole layer in registers (most in
+//   accumulation registers, which the MFMA reads directly) for the whole
+//   march, so the fully unrolled K loop reads only A from LDS, one K block
+//   ahead of the MFMAs that use it.  The voxel stride in LDS is an odd
+//   number of 16-byte units so the 16 lanes of a fragment read hit distinct
+//   banks.  The epilogue adds the bias and applies GELU (hidden layers), or
+//   multiplies by GELU'(z) (backward).
 //   The backward pass for the input reuses it with the flipped, transposed
 //   kernel.
 // k_dw: weight gradient dW[co][tap][ci] = sum_v G[v][co] X[v+off][ci] as
@@ -52,17 +54,28 @@ constexpr int XS = LW + 7;                // dW: LDS row stride of X (73, odd)
 template <int CIN>
 constexpr int cstride() { return ((CIN / 8) % 2 == 0) ? CIN + 8 : CIN; }
 
-// LDS row stride (bf16) of a weight panel with rows of kp = 32 x K-blocks.
-__host__ __device__ constexpr int wstride(int kp) { return ((kp / 8) % 2 == 0) ? kp + 8 : kp; }
 
 __device__ __forceinline__ int wrapn(int x, int n) {
     int r = x % n;
     return r < 0 ? r + n : r;
 }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.26
+// (absolute error below 1.5e-7, far under the BF16 rounding of the
+// activations): branch-free, one hardware reciprocal and one exponential.
+__device__ __forceinline__ float erf_as(float u) {
+    const float s = fabsf(u);
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.3275911f, s, 1.0f));
+    float q = __builtin_fmaf(1.061405429f, t, -1.453152027f);
+    q = __builtin_fmaf(q, t, 1.421413741f);
+    q = __builtin_fmaf(q, t, -0.284496736f);
+    q = __builtin_fmaf(q, t, 0.254829592f);
+    const float e = 1.0f - q * t * __expf(-s * s);
+    return copysignf(e, u);
+}
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float x) {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.39894228040143268f;
+    return 0.5f * (1.0f + erf_as(x * 0.70710678118654752f)) + x * __expf(-0.5f * x * x) * 0.39894228040143268f;
 }
 
 // EPI 0: out32 = acc + bias.  EPI 1: zout = acc + bias, out16 = gelu(zout).
@@ -70,7 +83,7 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // EPI 4: out16 = gelu(acc + bias) (inference: the pre-activation, which only
 // the backward pass reads, is not stored).
 template <int CIN, int NT, int EPI>
-__global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const bf16 *__restrict__ wt, int kp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_conv(const bf16 *__restrict__ x, const bf16 *__restrict__ wt, int kp,
                                               const float *__restrict__ bias, const float *__restrict__ zprev,
                                               bf16 *__restrict__ out16, float *__restrict__ out32,
                                               float *__restrict__ zout, int cout, int ostride, int nx, int ny,
@@ -82,8 +95,6 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
     constexpr int KTOT = 27 * CIN, KB = (KTOT + 31) / 32;
     extern __shared__ uint4 lds_raw[];
     bf16 *const lds = reinterpret_cast<bf16 *>(lds_raw);
-    bf16 *const wl = lds + 3 * PLANE;  // weight panel, 16 NT rows of stride KPL
-    const int KPL = wstride(kp);
 
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6, lr = lane & 15, lg = lane >> 4;
     const int ntx = (nx + CTX - 1) / CTX, nty = (ny + CTY - 1) / CTY;
@@ -94,18 +105,31 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
     const int zb = tz * kchunk, ze = min(nz, zb + kchunk);
     if (zb >= nz) return;
 
+    // this thread's 16-byte pieces of an input plane with halo: offsets in a
+    // plane of x (-1: no piece) and in an LDS plane, computed once
+    int goff[NLOAD], loff[NLOAD];
+#pragma unroll
+    for (int i = 0; i < NLOAD; ++i) {
+        const int c = t + 256 * i;
+        goff[i] = -1;
+        loff[i] = 0;
+        if (c < LW * LH * NCH) {
+            const int vox = c / NCH, part = c - vox * NCH;
+            const int row = vox / LW, col = vox - row * LW;
+            const int gy = wrapn(y0 + row - 1, ny), gx = wrapn(x0 + col - 1, nx);
+            goff[i] = (gy * nx + gx) * CIN + part * 8;
+            loff[i] = vox * CS + part * 8;
+        }
+    }
+    const long long pstride = (long long)nx * ny * CIN;
     uint4 pre[NLOAD];
     auto fetch = [&](int zz) {
+        const bool in = zz >= 0 && zz < nz;
+        const bf16 *xz = x + (in ? zz : 0) * pstride;
 #pragma unroll
         for (int i = 0; i < NLOAD; ++i) {
-            const int c = t + 256 * i;
             uint4 v = make_uint4(0, 0, 0, 0);
-            if (c < LW * LH * NCH && zz >= 0 && zz < nz) {
-                const int vox = c / NCH, part = c - vox * NCH;
-                const int row = vox / LW, col = vox - row * LW;
-                const int gy = wrapn(y0 + row - 1, ny), gx = wrapn(x0 + col - 1, nx);
-                v = *reinterpret_cast<const uint4 *>(x + ((((long long)zz * ny + gy) * nx + gx) * CIN + part * 8));
-            }
+            if (in && goff[i] >= 0) v = *reinterpret_cast<const uint4 *>(xz + goff[i]);
             pre[i] = v;
         }
     };
@@ -113,19 +137,17 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
     auto store = [&](int zz) {
         bf16 *pl = lds + slot(zz) * PLANE;
 #pragma unroll
-        for (int i = 0; i < NLOAD; ++i) {
-            const int c = t + 256 * i;
-            if (c < LW * LH * NCH) {
-                const int vox = c / NCH, part = c - vox * NCH;
-                *reinterpret_cast<uint4 *>(pl + vox * CS + part * 8) = pre[i];
-            }
-        }
+        for (int i = 0; i < NLOAD; ++i)
+            if (goff[i] >= 0) *reinterpret_cast<uint4 *>(pl + loff[i]) = pre[i];
     };
-    for (int c = t; c < 16 * NT * (kp / 8); c += 256) {
-        const int row = c / (kp / 8), part = c - row * (kp / 8);
-        *reinterpret_cast<uint4 *>(wl + row * KPL + part * 8) =
-            *reinterpret_cast<const uint4 *>(wt + (long long)row * kp + part * 8);
-    }
+    // the layer's B fragments (weights) of this lane, held in registers
+    // for the whole march: every plane uses all of them
+    bf16x8 wb[KB][NT];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+            wb[kb][nt] = *reinterpret_cast<const bf16x8 *>(wt + (long long)(16 * nt + lr) * kp + kb * 32 + lg * 8);
     for (int zz = zb - 1; zz <= zb + 1; ++zz) {
         fetch(zz);
         store(zz);
@@ -140,10 +162,10 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 3
-        for (int kb = 0; kb < KB; ++kb) {
+        // operand fragments of K block kb (A: 4 M tiles of this wavefront's
+        // row, B: the NT output-channel tiles)
+        auto frag = [&](int kb, bf16x8 *a) __attribute__((always_inline)) {
             const int k0 = kb * 32 + lg * 8;
-            bf16x8 a[4], b[NT];
             if (k0 < KTOT) {
                 const int tap = k0 / CIN, ci0 = k0 - tap * CIN;
                 const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
@@ -154,36 +176,60 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ x, const 
 #pragma unroll
                 for (int mt = 0; mt < 4; ++mt) a[mt] = bf16x8{};
             }
+        };
+        // K loop, software-pipelined: the LDS reads of block kb + 1 are
+        // issued before the MFMAs of block kb, which cover their latency
+        // (one wavefront per SIMD: nothing else would)
+        bf16x8 fa[2][4];
+        frag(0, fa[0]);
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                b[nt] = *reinterpret_cast<const bf16x8 *>(wl + (16 * nt + lr) * KPL + k0);
+        for (int kb = 0; kb < KB; ++kb) {
+            const int c = kb & 1;
+            if (kb + 1 < KB) frag(kb + 1, fa[c ^ 1]);
 #pragma unroll
             for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt], b[nt], acc[mt][nt], 0, 0, 0);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[c][mt], wb[kb][nt], acc[mt][nt], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
+        // epilogue: lane (lr, lg) holds voxels x0 + 16 mt + 4 lg + r of this
+        // wavefront's row for channel 16 nt + lr.  The row and channel
+        // checks are per lane; the column check only in a ragged tile.
         const int gy = y0 + wv;
+        const long long rowo = (((long long)z * ny + gy) * nx + x0) * ostride;
+        auto emit = [&](float v, int co, float bco, long long o) __attribute__((always_inline)) {
+            if (EPI == 0) out32[o] = v + bco;
+            if (EPI == 1 || EPI == 4) {
+                const float zz = v + bco;
+                if (EPI == 1) zout[o] = zz;
+                out16[o] = (bf16)gelu(zz);
+            }
+            if (EPI == 2) out16[o] = (bf16)(v * gelu_grad(zprev[o]));
+            if (EPI == 3) out32[o] = v;
+        };
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = 16 * nt + lr;
+            if (gy < ny && co < cout) {
+                const float bco = (EPI == 0 || EPI == 1 || EPI == 4) ? bias[co] : 0.f;
+                if (x0 + CTX <= nx) {
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
+                    for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gx = x0 + 16 * mt + 4 * lg + r, co = 16 * nt + lr;
-                    if (gx < nx && gy < ny && co < cout) {
-                        const long long o = (((long long)z * ny + gy) * nx + gx) * ostride + co;
-                        const float v = acc[mt][nt][r];
-                        if (EPI == 0) out32[o] = v + bias[co];
-                        if (EPI == 1 || EPI == 4) {
-                            const float zz = v + bias[co];
-                            if (EPI == 1) zout[o] = zz;
-                            out16[o] = (bf16)gelu(zz);
+                        for (int r = 0; r < 4; ++r)
+                            emit(acc[mt][nt][r], co, bco, rowo + (long long)(16 * mt + 4 * lg + r) * ostride + co);
+                } else {
+#pragma unroll
+                    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int xo = 16 * mt + 4 * lg + r;
+                            if (x0 + xo < nx) emit(acc[mt][nt][r], co, bco, rowo + (long long)xo * ostride + co);
                         }
-                        if (EPI == 2) out16[o] = (bf16)(v * gelu_grad(zprev[o]));
-                        if (EPI == 3) out32[o] = v;
-                    }
                 }
+            }
+        }
         __syncthreads();
         if (more) store(z + 2);
         __syncthreads();
@@ -489,8 +535,10 @@ static int kchunk_for(int nx, int ny, int nz, int target, int *nblocks) {
 template <int CIN, int NT, int EPI>
 static hipError_t conv(hipStream_t s, const bf16 *x, const bf16 *w, int kp, const float *bias, const float *zprev,
                        bf16 *out16, float *out32, float *zout, int cout, int ostride, int nx, int ny, int nz) {
-    const int lds = (3 * LW * LH * cstride<CIN>() + 16 * NT * wstride(kp)) * (int)sizeof(bf16);
+    const int lds = 3 * LW * LH * cstride<CIN>() * (int)sizeof(bf16);
+    if (kp != 32 * ((27 * CIN + 31) / 32)) return hipErrorInvalidValue;  // the kernel's K blocks
     if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if ((long long)nx * ny * CIN > 0x7fffffffLL) return hipErrorInvalidValue;  // in-plane offsets are 32-bit
     static int attr = 0;
     if (lds > attr) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv<CIN, NT, EPI>),
@@ -498,11 +546,21 @@ static hipError_t conv(hipStream_t s, const bf16 *x, const bf16 *w, int kp, cons
         if (e != hipSuccess) return e;
         attr = lds;
     }
-    // as many workgroups as fit the CUs at once (LDS bound): each streams a
-    // longer z chunk, which amortises its prologue (three planes + weights)
-    const int per_cu = std::max(1, (160 * 1024) / lds);
+    // as many workgroups as fit the CUs at once (LDS and registers): each
+    // streams a longer z chunk, which amortises its prologue (three planes +
+    // the weights)
+    static int resident = 0;
+    if (!resident) {
+        int per_cu = 0, ncu = 0, dev = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&k_conv<CIN, NT, EPI>), 256, lds);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        resident = std::max(1, per_cu) * std::max(1, ncu);
+    }
     int nb;
-    const int kc = kchunk_for(nx, ny, nz, 256 * per_cu, &nb);
+    const int kc = kchunk_for(nx, ny, nz, resident, &nb);
     hipLaunchKernelGGL((k_conv<CIN, NT, EPI>), dim3(nb), dim3(256), lds, s, x, w, kp, bias, zprev, out16, out32, zout,
                        cout, ostride, nx, ny, nz, kc);
     return hipGetLastError();
