@@ -457,6 +457,7 @@ def run(nxc=32, nyc=32, nzc=32, factor=4, dev_factor=1, until_mach=0.05, develop
     net.close()
     rep = {
         "fine": list(ex.g.fine_shape), "coarse": [nxc, nyc, nzc], "dev_factor": dev_factor,
+        "physics": {k: kw[k] for k in ("contrast", "ra", "pr", "pm", "ta") if k in kw},
         "develop_steps": [n_dev, develop], "develop_mach": mach_dev, "develop_wall_s": dev_wall,
         "samples": nsamples, "every": every, "sample_times": T, "horizon_fine_steps": horizon, "snapshots": nsnap,
         "window_times": times, "coarse_steps": m * nsnap,
@@ -494,6 +495,10 @@ if __name__ == "__main__":
     ap.add_argument("--snapshots", type=int, default=8)
     ap.add_argument("--iters", type=int, default=6000)
     ap.add_argument("--ampt", type=float, default=5e-2)
+    ap.add_argument("--contrast", type=float, default=10.0, help="density contrast of the layer")
+    ap.add_argument("--ra", type=float, default=1e5, help="Rayleigh number")
+    ap.add_argument("--pm", type=float, default=1.0, help="magnetic Prandtl number")
+    ap.add_argument("--ta", type=float, default=1e4, help="Taylor number (vertical rotation axis)")
     ap.add_argument("--out", default=None)
     ap.add_argument("--time-net", type=int, nargs=3, default=None, metavar=("NX", "NY", "NZ"),
                     help="only time the network's forward pass (He-initialised weights) on an NX x NY x NZ box")
@@ -506,7 +511,8 @@ if __name__ == "__main__":
         sys.exit(0)
     r = run(*a.coarse, dev_factor=a.dev_factor, until_mach=a.until_mach, develop_max=a.develop_max,
             develop=a.develop, nsamples=a.samples, every=a.every, horizon=a.horizon, nsnap=a.snapshots,
-            iters=a.iters, ampt=a.ampt, log=lambda *x: print(*x, file=sys.stderr, flush=True))
+            iters=a.iters, ampt=a.ampt, contrast=a.contrast, ra=a.ra, pm=a.pm, ta=a.ta,
+            log=lambda *x: print(*x, file=sys.stderr, flush=True))
     s = json.dumps(r)
     if a.out:
         open(a.out, "w").write(s + "\n")
