@@ -103,6 +103,18 @@ int mhd_set_coordinates(mhd_t *h, const double *exx, const double *wyy, const do
  * length NZ.  Call after mhd_set_metrics and before mhd_upload. */
 int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0);
 
+/* MHD_MIXED only (no counterpart in the reference): re-centres the
+ * background of the interior planes on the current horizontal means of RO
+ * and TT and re-expresses the stored state about it, in place; the result
+ * is what mhd_download, mhd_set_background(means) and mhd_upload would
+ * give.  A background that has drifted from the flow's means leaves a
+ * large horizontally uniform part in the stored pressure perturbation, and
+ * its FP32 rounding then weighs on the horizontal pressure gradient.
+ * mhd_set_rebase_interval(h, n) does it after every n-th step of mhd_step
+ * and mhd_steps (0: never, the default); mhd_time_steps never does. */
+int mhd_rebase(mhd_t *h);
+int mhd_set_rebase_interval(mhd_t *h, int nsteps);
+
 /* Uploads the prognostic arrays RU,RV,RW,RO,TT[,BX,BY,BZ] of /BIG/.  The
  * x/y ghosts are ignored (the library wraps periodically, which is what
  * COMMUNICATE establishes, 3dmhdsub.f:3131-3138); the z ghost planes are

@@ -73,6 +73,8 @@ def load(path: str | None = None):
     lib.mhd_create.argtypes = [C.POINTER(MhdDesc), C.POINTER(C.c_void_p)]
     lib.mhd_set_metrics.argtypes = [C.c_void_p] + [_P] * 11
     lib.mhd_set_background.argtypes = [C.c_void_p, _P, _P]
+    lib.mhd_rebase.argtypes = [C.c_void_p]
+    lib.mhd_set_rebase_interval.argtypes = [C.c_void_p, C.c_int]
     lib.mhd_set_coordinates.argtypes = [C.c_void_p, _P, _P, _P]
     lib.mhd_upload.argtypes = [C.c_void_p] + [_P] * 8
     lib.mhd_download.argtypes = [C.c_void_p] + [_P] * 8
@@ -147,7 +149,7 @@ def load(path: str | None = None):
         getattr(lib, fn).restype = C.c_int
     for fn in ("mhd_create", "mhd_set_metrics", "mhd_set_coordinates", "mhd_set_background", "mhd_upload", "mhd_download", "mhd_step", "mhd_steps",
                "mhd_get_clock", "mhd_set_clock", "mhd_fluxes", "mhd_time_steps", "mhd_set_fixed_dt", "mhd_box_filter",
-               "mhd_flux_profiles"):
+               "mhd_flux_profiles", "mhd_rebase", "mhd_set_rebase_interval"):
         getattr(lib, fn).restype = C.c_int
     _lib = lib
     return lib
@@ -243,6 +245,15 @@ class Box:
             raise ValueError(f"background profiles must have {self.shape[2]} planes")
         self._check(self.lib.mhd_set_background(self.h, _ptr(ro0), _ptr(tt0)))
         self._have_bg = True
+
+    def rebase(self):
+        """Mixed form: re-centre the background on the current horizontal
+        means of rho and T (the state is re-expressed about it in place)."""
+        self._check(self.lib.mhd_rebase(self.h))
+
+    def set_rebase_interval(self, nsteps: int):
+        """Mixed form: rebase() after every nsteps-th step (0: never)."""
+        self._check(self.lib.mhd_set_rebase_interval(self.h, int(nsteps)))
 
     def upload(self, fields: dict):
         if self.precision == "mixed" and not self._have_bg:

@@ -40,6 +40,12 @@ struct mhd_ctx {
     float *bgf = nullptr;               // mixed: FP32 background arrays
     PlaneM *bgp = nullptr;              // mixed: per-plane records of the stage kernel
     Background bg{};
+    std::vector<double> bg_ro, bg_tt;   // mixed: the background profiles as set (npz + 2 planes)
+    double *bgd2 = nullptr;             // mixed: second set of background tables (mhd_rebase)
+    float *bgf2 = nullptr;
+    PlaneM *bgp2 = nullptr;
+    double *prof = nullptr;             // mixed: plane means for mhd_rebase [3][npz]
+    int rebase_every = 0, since_rebase = 0;
     double *f = nullptr;                // right-hand side (mhd_fluxes only)
     double *stage = nullptr;            // one padded field for host transfers
     double *met = nullptr;              // 11 metric arrays
@@ -72,6 +78,8 @@ struct mhd_ctx {
 
 static char g_create_err[512];
 
+static int ready(mhd_t *h, const char *who);
+
 static int fail(mhd_t *h, int code, const char *fmt, ...) {
     char *buf = h ? h->err : g_create_err;
     va_list ap;
@@ -103,9 +111,9 @@ static void free_all(mhd_t *h) {
     if (!h) return;
     (void)hipSetDevice(h->dev);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void *bufs[21] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->rr, h->bgd, h->bgf, h->bgp,
+    void *bufs[25] = {h->q[0], h->q[1], h->z, h->hq[0], h->hq[1], h->dr, h->rr, h->bgd, h->bgf, h->bgp,
                       h->f, h->stage, h->met, h->sc, h->force, h->force_on, h->cx, h->nrm,
-                      h->optmem, h->rs, h->ftacc};
+                      h->optmem, h->rs, h->ftacc, h->bgd2, h->bgf2, h->bgp2, h->prof};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -332,8 +340,78 @@ extern "C" int mhd_set_background(mhd_t *h, const double *ro0, const double *tt0
         h->bg.force = h->force;
         h->bg.force_on = h->force_on;
     }
+    h->bg_ro.assign(ro0, ro0 + h->p.npz + 2);
+    h->bg_tt.assign(tt0, tt0 + h->p.npz + 2);
+    h->since_rebase = 0;
     h->have_bg = true;
     h->have_state = false;
+    return MHD_OK;
+}
+
+// Re-centres the mixed form's background on the current horizontal means of
+// rho and T (interior planes; the z ghost planes keep theirs) and re-expresses
+// the stored rho and T about it in place, as download + set_background +
+// upload would, without the host round trip of the state.
+static int rebase(mhd_t *h) {
+    const Params &p = h->p;
+    const int nz = p.npz;
+    const size_t nzp = (size_t)nz + 2;
+    if (!h->bgd2) {
+        hipError_t e = hipMalloc(&h->bgd2, BG_DBL * nzp * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&h->bgf2, BG_FLT * nzp * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc(&h->bgp2, nzp * sizeof(PlaneM));
+        if (e == hipSuccess) e = hipMalloc(&h->prof, 3 * (size_t)nz * sizeof(double));
+        if (e != hipSuccess) return fail(h, MHD_ERR_HIP, "mhd_rebase: allocation: %s", hipGetErrorString(e));
+    }
+    std::vector<double> prof(3 * (size_t)nz);
+    HIPCHK(h, launch_flux_profiles_mixed(p, h->hq[h->cur], h->rr, h->bg, h->prof, h->stream));
+    HIPCHK(h, hipMemcpyAsync(prof.data(), h->prof, prof.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::vector<double> ro0 = h->bg_ro, tt0 = h->bg_tt;
+    for (int k = 1; k <= nz; ++k) {
+        tt0[k] = prof[k - 1];
+        ro0[k] = prof[nz + k - 1];
+    }
+    std::vector<double> d;
+    std::vector<float> f;
+    std::vector<PlaneM> pl;
+    const int bad = background_tables(p, h->dzzdz.data(), h->d2zzdz2.data(), h->ddz.data(), h->rkapa.data(),
+                                      h->dkapa.data(), ro0.data(), tt0.data(), d, f, pl);
+    if (bad >= 0) return fail(h, MHD_ERR_NONFINITE, "mhd_rebase: horizontal mean of rho[%d] <= 0", bad);
+    HIPCHK(h, hipMemcpyAsync(h->bgd2, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->bgf2, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->bgp2, pl.data(), pl.size() * sizeof(PlaneM), hipMemcpyHostToDevice, h->stream));
+    Background nb = background_view(nz, h->bgd2, h->bgf2);
+    nb.planes = h->bgp2;
+    nb.force = h->bg.force;
+    nb.force_on = h->bg.force_on;
+    HIPCHK(h, launch_rebase_mixed(p, h->bg, nb, h->hq[h->cur], h->rr, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::swap(h->bgd, h->bgd2);
+    std::swap(h->bgf, h->bgf2);
+    std::swap(h->bgp, h->bgp2);
+    h->bg = nb;
+    h->bg_ro.swap(ro0);
+    h->bg_tt.swap(tt0);
+    h->cfl_valid = false;  // the next step reduces the CFL limits of the re-expressed state
+    h->since_rebase = 0;
+    return MHD_OK;
+}
+
+extern "C" int mhd_rebase(mhd_t *h) {
+    int rc = ready(h, "mhd_rebase");
+    if (rc) return rc;
+    if (!h->mixed) return fail(h, MHD_ERR_STATE, "mhd_rebase: only for MHD_MIXED handles");
+    HIPCHK(h, hipSetDevice(h->dev));
+    return rebase(h);
+}
+
+extern "C" int mhd_set_rebase_interval(mhd_t *h, int nsteps) {
+    if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_set_rebase_interval: null handle");
+    if (nsteps < 0) return fail(h, MHD_ERR_ARG, "mhd_set_rebase_interval: nsteps=%d < 0", nsteps);
+    if (nsteps && !h->mixed) return fail(h, MHD_ERR_STATE, "mhd_set_rebase_interval: only for MHD_MIXED handles");
+    h->rebase_every = nsteps;
+    h->since_rebase = 0;
     return MHD_OK;
 }
 
@@ -466,12 +544,19 @@ static hipError_t rhs_eval(mhd_t *h) {
     return e;
 }
 
+// After a step: the periodic re-centring of the mixed background, if set.
+static int after_step(mhd_t *h) {
+    if (!h->rebase_every || ++h->since_rebase < h->rebase_every) return MHD_OK;
+    return rebase(h);
+}
+
 extern "C" int mhd_step(mhd_t *h, double *dt, double *umach) {
     int rc = ready(h, "mhd_step");
     if (rc) return rc;
     HIPCHK(h, hipSetDevice(h->dev));
     HIPCHK(h, launches(h, nullptr));
-    return finish(h, dt, umach);
+    rc = finish(h, dt, umach);
+    return rc ? rc : after_step(h);
 }
 
 extern "C" int mhd_steps(mhd_t *h, int n, double *dt, double *umach) {
@@ -479,7 +564,15 @@ extern "C" int mhd_steps(mhd_t *h, int n, double *dt, double *umach) {
     if (rc) return rc;
     if (n < 0) return fail(h, MHD_ERR_ARG, "mhd_steps: n=%d", n);
     HIPCHK(h, hipSetDevice(h->dev));
-    for (int i = 0; i < n; ++i) HIPCHK(h, launches(h, nullptr));
+    for (int i = 0; i < n; ++i) {
+        HIPCHK(h, launches(h, nullptr));
+        if (h->rebase_every && ++h->since_rebase >= h->rebase_every) {
+            // the step's DT and UMACH are read (and checked) before the re-centring
+            rc = finish(h, dt, umach);
+            if (rc == MHD_OK) rc = rebase(h);
+            if (rc) return rc;
+        }
+    }
     return finish(h, dt, umach);
 }
 

@@ -227,6 +227,10 @@ hipError_t launch_pack_mixed(const Params &p, const double *padded, int field, c
                              unsigned *rreg, hipStream_t s);
 hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const unsigned *rreg, int field,
                                const Background &bg, double *padded, hipStream_t s);
+// Re-expresses rho and T of the interior planes about the background nb
+// instead of ob, in place (mhd_rebase).
+hipError_t launch_rebase_mixed(const Params &p, const Background &ob, const Background &nb, float *hreg,
+                               unsigned *rreg, hipStream_t s);
 
 hipError_t launch_stage_batch_mixed(int mode, int tile_x, int tile_y, int lmag, const BoxDev *boxes,
                                     const StageItem *items, int nitems, int grid, int a, hipStream_t s);

@@ -725,6 +725,32 @@ __global__ void k_unpack_m(const Params p, const float *__restrict__ hreg, const
     }
 }
 
+// Re-centring of the background (mhd_rebase): rho and T of every interior
+// point, reconstructed about the old background ob exactly as k_unpack_m
+// does, are split about the new background nb exactly as k_pack_m does
+// (rho' before T', whose stored P' uses the new rho' + R).  The other fields
+// do not depend on the background; the z ghost planes keep their background.
+__global__ void k_rebase_m(const Params p, const Background ob, const Background nb, float *__restrict__ hreg,
+                           unsigned *__restrict__ rreg) {
+    const long long fs = p.fsize, n = (long long)p.npz * p.plane;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+         v += (long long)gridDim.x * blockDim.x) {
+        const long long idx = p.plane + v;
+        const int k = (int)(idx / p.plane);
+        const float h3 = hreg[3 * fs + idx], h4 = hreg[4 * fs + idx];
+        const double ro = ((double)h3 + ob.ro[k]) + (double)r_get(rreg, fs, 3, idx);
+        const double tt = (ob.tt[k] + (double)trep(ob.ro32[k], ob.tt32[k], h3, h4)) + (double)r_get(rreg, fs, 4, idx);
+        const double rp = ro - nb.ro[k];
+        const float n3 = (float)rp;
+        hreg[3 * fs + idx] = n3;
+        r_put(rreg, fs, 3, idx, (float)(rp - (double)n3));
+        const double tp = tt - nb.tt[k];
+        const float n4 = (float)pprime(nb.ro[k], nb.tt[k], (double)n3 + (double)r_get(rreg, fs, 3, idx), tp);
+        hreg[4 * fs + idx] = n4;
+        r_put(rreg, fs, 4, idx, (float)(tp - (double)trep(nb.ro32[k], nb.tt32[k], n3, n4)));
+    }
+}
+
 // Network inputs of the closure from the state (h, R) about the background.
 __global__ void k_closure_inputs(const Params p, const Background bg, const float *__restrict__ h,
                                  const unsigned *__restrict__ rr, const float *__restrict__ nrm,
@@ -868,6 +894,15 @@ hipError_t launch_unpack_mixed(const Params &p, const float *hreg, const unsigne
     long long nb = (n + 255) / 256;
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(k_unpack_m, dim3((unsigned)nb), dim3(256), 0, s, p, hreg, rreg, field, bg, padded);
+    return hipGetLastError();
+}
+
+hipError_t launch_rebase_mixed(const Params &p, const Background &ob, const Background &nb, float *hreg,
+                               unsigned *rreg, hipStream_t s) {
+    const long long n = (long long)p.npz * p.plane;
+    long long nb_ = (n + 255) / 256;
+    if (nb_ > 8192) nb_ = 8192;
+    hipLaunchKernelGGL(k_rebase_m, dim3((unsigned)nb_), dim3(256), 0, s, p, ob, nb, hreg, rreg);
     return hipGetLastError();
 }
 

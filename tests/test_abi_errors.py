@@ -51,6 +51,9 @@ def test_null_arguments():
     assert lib.mhd_create(None, None) == H.pkg().MHD_ERR_ARG
     assert lib.mhd_step(None, None, None) == H.pkg().MHD_ERR_ARG
     assert b"null" in lib.mhd_last_error(None)
+    assert lib.mhd_rebase(None) == H.pkg().MHD_ERR_ARG
+    assert lib.mhd_set_rebase_interval(None, 10) == H.pkg().MHD_ERR_ARG
+    assert b"null" in lib.mhd_last_error(None)
 
 
 @pytest.mark.gpu
