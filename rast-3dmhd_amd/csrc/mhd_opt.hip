@@ -4,8 +4,8 @@
 // reference's own summation order for an NPEY-slab decomposition: each slab
 // (rows j = s*NRY .. s*NRY+NRY-1 of a plane) is summed sequentially, rows
 // outer and points inner, then divided by FLOAT(NPX*NPY), and the slab
-// quotients are added in slab order.  One thread per (plane, slab) forms a
-// slab's sum; one workgroup combines them per plane.  The means feed
+// quotients are added in slab order.  One workgroup per (plane, slab) forms
+// a slab's sum; one workgroup combines them per plane.  The means feed
 //   ROM   density diffusion (ID, sub:1301-1371), with the ghost planes of RO
 //         holding the mirrored planes 2 and npz-1 (ID or LREM);
 //   TTM   Rempel conduction (LREM, sub:1536-1581) and the relaxation;
@@ -24,29 +24,67 @@ namespace rmhd {
 
 namespace {
 
-// Slab sums of plane k (0..npz+1) of RO (what & 1) and TT (what & 2).
-__global__ void k_slab_sums(const Params p, const double *__restrict__ q, double *__restrict__ part, int what) {
+// The reference's sums are one sequential chain per slab (rows outer, points
+// inner; the slab is a contiguous run of NRY*NPX points of the plane).  A
+// workgroup per (plane, slab) stages the run through LDS in chunks of
+// SUM_CHUNK values (coalesced, double-buffered) and its thread 0 adds them
+// in order, so the chain runs at the latency of an FP64 add instead of that
+// of a dependent global load.
+constexpr int SUM_T = 256, SUM_CHUNK = 1024;
+
+// sum + v[0] + v[1] + ... + v[m-1], added one by one in this order; the LDS
+// reads of a group of 8 are issued before its adds.
+__device__ __forceinline__ double chain_add(double sum, const double *v, int m) {
+    int i = 0;
+    for (; i + 8 <= m; i += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = v[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sum = sum + x[u];
+    }
+    for (; i < m; ++i) sum = sum + v[i];
+    return sum;
+}
+
+// Slab sums of plane k (0..npz+1) of RO (WHAT & 1) and TT (WHAT & 2).
+template <int WHAT>
+__global__ __launch_bounds__(SUM_T) void k_slab_sums(const Params p, const double *__restrict__ q,
+                                                     double *__restrict__ part) {
+    __shared__ double buf[2][2][SUM_CHUNK];
     const int npl = p.npz + 2, nsl = p.npey;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npl * nsl) return;
-    const int k = t / nsl, sl = t - k * nsl;
+    const int b = blockIdx.x;  // b = k * nsl + sl
+    const int k = b / nsl, sl = b - k * nsl;
     const int nry = p.npy / nsl;
-    const long long fs = p.fsize, pl = p.plane;
+    const long long fs = p.fsize, pl = p.plane, n = (long long)nry * p.npx;
     int kr = k;
     if (p.mirror && k == 0) kr = 2;
     if (p.mirror && k == p.npz + 1) kr = p.npz - 1;
-    const double *ro = q + 3 * fs + (long long)kr * pl;
-    const double *tt = q + 4 * fs + (long long)k * pl;
-    double sr = 0.0, st = 0.0;
-    for (int j = sl * nry; j < (sl + 1) * nry; ++j) {
-        const long long r = (long long)j * p.npx;
-        for (int i = 0; i < p.npx; ++i) {
-            if (what & 1) sr = sr + ro[r + i];
-            if (what & 2) st = st + tt[r + i];
+    const double *ro = q + 3 * fs + (long long)kr * pl + (long long)sl * nry * p.npx;
+    const double *tt = q + 4 * fs + (long long)k * pl + (long long)sl * nry * p.npx;
+    auto stage = [&](long long c0, int slot) {
+        for (int i = threadIdx.x; i < SUM_CHUNK; i += SUM_T) {
+            if (c0 + i < n) {
+                if (WHAT & 1) buf[slot][0][i] = ro[c0 + i];
+                if (WHAT & 2) buf[slot][1][i] = tt[c0 + i];
+            }
         }
+    };
+    // RO's chain on thread 0, TT's on thread 64 (another SIMD)
+    const int f = threadIdx.x == 0 ? 0 : threadIdx.x == 64 ? 1 : -1;
+    const bool chain = f >= 0 && (WHAT & (1 << f));
+    double sum = 0.0;
+    const long long nch = (n + SUM_CHUNK - 1) / SUM_CHUNK;
+    stage(0, 0);
+    __syncthreads();
+    for (long long c = 0; c < nch; ++c) {
+        const int slot = (int)(c & 1);
+        if (c + 1 < nch) stage((c + 1) * SUM_CHUNK, slot ^ 1);
+        if (chain) sum = chain_add(sum, buf[slot][f], (int)min((long long)SUM_CHUNK, n - c * SUM_CHUNK));
+        __syncthreads();
     }
-    part[t] = sr;
-    part[(long long)npl * nsl + t] = st;
+    if (chain) part[(long long)f * npl * nsl + b] = sum;
+    else if (f >= 0) part[(long long)f * npl * nsl + b] = 0.0;
 }
 
 // The slab quotients of plane k added in slab order.
@@ -84,30 +122,44 @@ __global__ void k_plane_means(const Params p, const Metrics m, const double *__r
 }
 
 // Slab sums of the convective-flux summand -RW*(2.5*(TT-TTM) + 0.5*|u|^2) of
-// planes k = 1..npz+1 (sub:1653-1662); u = RU*(1/RO) as FLUXES forms it.
-__global__ void k_fconm_sums(const Params p, const double *__restrict__ q, const double *__restrict__ ttm,
-                             double *__restrict__ part) {
+// planes k = 1..npz+1 (sub:1653-1662); u = RU*(1/RO) as FLUXES forms it.  The
+// summands are formed by the staging threads, the chain by thread 0, as in
+// k_slab_sums.
+__global__ __launch_bounds__(SUM_T) void k_fconm_sums(const Params p, const double *__restrict__ q,
+                                                      const double *__restrict__ ttm, double *__restrict__ part) {
+    __shared__ double buf[2][SUM_CHUNK];
     const int nsl = p.npey;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (p.npz + 1) * nsl) return;
-    const int k = 1 + t / nsl, sl = t % nsl;
+    const int b = blockIdx.x;  // b = (k - 1) * nsl + sl
+    const int k = 1 + b / nsl, sl = b % nsl;
     const int nry = p.npy / nsl;
-    const long long fs = p.fsize, pl = p.plane;
+    const long long fs = p.fsize, pl = p.plane, n = (long long)nry * p.npx;
+    const long long so = (long long)sl * nry * p.npx;
     const int kr = (p.mirror && k == p.npz + 1) ? p.npz - 1 : k;
-    const double *ru = q + (long long)k * pl, *rv = q + fs + (long long)k * pl, *rw = q + 2 * fs + (long long)k * pl;
-    const double *ro = q + 3 * fs + (long long)kr * pl, *tt = q + 4 * fs + (long long)k * pl;
+    const double *ru = q + (long long)k * pl + so, *rv = q + fs + (long long)k * pl + so;
+    const double *rw = q + 2 * fs + (long long)k * pl + so;
+    const double *ro = q + 3 * fs + (long long)kr * pl + so, *tt = q + 4 * fs + (long long)k * pl + so;
     const double tm = ttm[k];
-    double sum = 0.0;
-    for (int j = sl * nry; j < (sl + 1) * nry; ++j) {
-        const long long r = (long long)j * p.npx;
-        for (int i = 0; i < p.npx; ++i) {
-            const long long o = r + i;
-            const double ri = 1.0 / ro[o];
-            const double uu = ru[o] * ri, vv = rv[o] * ri, ww = rw[o] * ri;
-            sum = sum + (-(rw[o] * (2.5 * (tt[o] - tm) + 0.5 * (uu * uu + vv * vv + ww * ww))));
+    auto stage = [&](long long c0, int slot) {
+        for (int i = threadIdx.x; i < SUM_CHUNK; i += SUM_T) {
+            const long long o = c0 + i;
+            if (o < n) {
+                const double ri = 1.0 / ro[o];
+                const double uu = ru[o] * ri, vv = rv[o] * ri, ww = rw[o] * ri;
+                buf[slot][i] = -(rw[o] * (2.5 * (tt[o] - tm) + 0.5 * (uu * uu + vv * vv + ww * ww)));
+            }
         }
+    };
+    double sum = 0.0;
+    const long long nch = (n + SUM_CHUNK - 1) / SUM_CHUNK;
+    stage(0, 0);
+    __syncthreads();
+    for (long long c = 0; c < nch; ++c) {
+        const int slot = (int)(c & 1);
+        if (c + 1 < nch) stage((c + 1) * SUM_CHUNK, slot ^ 1);
+        if (threadIdx.x == 0) sum = chain_add(sum, buf[slot], (int)min((long long)SUM_CHUNK, n - c * SUM_CHUNK));
+        __syncthreads();
     }
-    part[k * nsl + sl] = sum;
+    if (threadIdx.x == 0) part[k * nsl + sl] = sum;
 }
 
 // One thread: TFAC, FCONM, FRADM, ALPHA and the per-rank integral CORRECT,
@@ -176,12 +228,14 @@ hipError_t launch_opt_means(const Params &p, const Metrics &m, const double *q, 
     const int what = (p.id != 0 ? 1 : 0) | ((p.lrem || p.rlax > 0.0) ? 2 : 0);
     if (what) {
         const int n = npl * p.npey;
-        hipLaunchKernelGGL(k_slab_sums, dim3((n + 63) / 64), dim3(64), 0, s, p, q, o.part, what);
+        if (what == 1) hipLaunchKernelGGL(k_slab_sums<1>, dim3(n), dim3(SUM_T), 0, s, p, q, o.part);
+        else if (what == 2) hipLaunchKernelGGL(k_slab_sums<2>, dim3(n), dim3(SUM_T), 0, s, p, q, o.part);
+        else hipLaunchKernelGGL(k_slab_sums<3>, dim3(n), dim3(SUM_T), 0, s, p, q, o.part);
         hipLaunchKernelGGL(k_plane_means, dim3(1), dim3(256), 0, s, p, m, o.part, o.rom, o.wwz, o.ttm, o.hrad, what);
     }
     if (p.rlax > 0.0) {
         const int n = (p.npz + 1) * p.npey;
-        hipLaunchKernelGGL(k_fconm_sums, dim3((n + 63) / 64), dim3(64), 0, s, p, q, o.ttm, o.part);
+        hipLaunchKernelGGL(k_fconm_sums, dim3(n), dim3(SUM_T), 0, s, p, q, o.ttm, o.part);
         hipLaunchKernelGGL(k_relax, dim3(1), dim3(64), 0, s, p, m, o.zee, o.part, o.ttm, o.work, sc, o.rs, o.radd);
     }
     return hipGetLastError();
