@@ -25,7 +25,8 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 // address space, so the compiler issues a scalar load.  A vector load here
 // would share the in-order vector-memory counter with the plane prefetch
 // and the wait for it would also wait for every load issued before it.
-// The tables are written only by host copies between kernels.
+// The tables are written only between kernels (host copies, or earlier
+// kernels on the stream, such as the plane means of the optional branches).
 template <class T>
 __device__ __forceinline__ T uload(const T *p, int i) {
     return ((const __attribute__((address_space(4))) T *)p)[i];

@@ -239,8 +239,11 @@ __device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, co
     // CFL accumulators of the new state (stage 3 only)
     double c_mach = 0.0, c_w1 = INFINITY, c_w2 = INFINITY, c_w3 = INFINITY, c_w4 = INFINITY;
     const double c_ddx = MODE == 3 ? m.ddx[gix] : 0.0, c_ddy = MODE == 3 ? m.ddy[gjy] : 0.0;
+    // the embedded heat loss's per-column Gaussians, read once
+    const double c_gx = (OPT && p.itc == 2) ? p.gx[gix] : 0.0, c_gy = (OPT && p.itc == 2) ? p.gy[gjy] : 0.0;
     // the per-column metrics arrive before the march, not inside its first plane
     settle(dx), settle(d2x), settle(dy), settle(d2y), settle(c_ddx), settle(c_ddy);
+    if (OPT) settle(c_gx), settle(c_gy);
 
     // One plane of the march.  BND instantiates the boundary-plane form
     // (one-sided z stencils, extra planes read directly); the interior form
@@ -319,19 +322,19 @@ __device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, co
             fr = fr - wz;
             if (OPT && p.id != 0) {
                 // diffusion of the density perturbations W1 = RO - ROM (sub:1319-1371)
-                const double romk = p.rom[k];
+                const double romk = uload(p.rom, k);
                 const double d0 = ro0 - romk;
                 if (p.dh != 0.0) {
                     const double dxp = CUR(S_RO, 1) - romk, dxm = CUR(S_RO, -1) - romk;
                     const double dyp = CUR(S_RO, SW) - romk, dym = CUR(S_RO, -SW) - romk;
                     const double ww2 = (dxp - dxm) * HX * d2x + (dxp - 2.0 * d0 + dxm) * H2X * dx * dx;
                     const double ww3 = (dyp - dym) * tmpy1 + (dyp - 2.0 * d0 + dym) * tmpy2;
-                    fr = fr + (ww2 + ww3) * (p.dh * ORE * p.wwz[k]);
+                    fr = fr + (ww2 + ww3) * (p.dh * ORE * uload(p.wwz, k));
                 }
                 if (p.dv != 0.0) {
-                    const double dzp = NXT(S_RO) - p.rom[k + 1], dzm = rm_.v[S_RO] - p.rom[k - 1];
+                    const double dzp = NXT(S_RO) - uload(p.rom, k + 1), dzm = rm_.v[S_RO] - uload(p.rom, k - 1);
                     const double ww2 = (dzp - dzm) * tmpz1 + (dzp - 2.0 * d0 + dzm) * tmpz2;
-                    fr = fr + ww2 * (p.dv * ORE * p.wwz[k]);
+                    fr = fr + ww2 * (p.dv * ORE * uload(p.wwz, k));
                 }
             }
 
@@ -370,13 +373,13 @@ __device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, co
             if (OPT && p.lrem) {
                 // of the perturbations W1 = TT - TTM only, plus the mean profile's
                 // radiative heating HRAD (sub:1536-1581)
-                const double tc = p.t_cond, tmk = p.ttm[k];
+                const double tc = p.t_cond, tmk = uload(p.ttm, k);
                 const double e0 = tt0 - tmk, exp_ = txp - tmk, exm = txm - tmk, eyp = typ - tmk, eym = tym - tmk;
-                const double ezp = tzp - p.ttm[k + 1], ezm = tzm - p.ttm[k - 1];
+                const double ezp = tzp - uload(p.ttm, k + 1), ezm = tzm - uload(p.ttm, k - 1);
                 ft = ft + ((exp_ - exm) * HX * d2x + (exp_ - 2.0 * e0 + exm) * H2X * dx * dx) * rinv * tc;
                 ft = ft + ((eyp - eym) * tmpy1 + (eyp - 2.0 * e0 + eym) * tmpy2) * rinv * tc;
                 ft = ft + ((ezp - ezm) * tmpz1 + (ezp - 2.0 * e0 + ezm) * tmpz2) * rinv * tc;
-                ft = ft + rinv * tc * p.hrad[k];
+                ft = ft + rinv * tc * uload(p.hrad, k);
             } else {
                 const double tc = p.t_cond;
                 ft = ft + ((txp - txm) * HX * d2x + (txp - 2.0 * tt0 + txm) * H2X * dx * dx) * rinv * tc * rk;
@@ -386,13 +389,13 @@ __device__ __forceinline__ void stage_tile(const Params &p, const Metrics &m, co
             }
             if (OPT) {
                 // flux relaxation (sub:1757-1765) and embedded heat loss (sub:1783-1806)
-                if (p.rlax > 0.0) ft = ft + p.radd[k];
+                if (p.rlax > 0.0) ft = ft + uload(p.radd, k);
                 if (p.itc == 2) {
                     const double HH = p.hh;
                     if (p.tc != 0.0)
-                        ft = ft - p.tp * OCV * 0.5 * p.ramp * rinv * p.gx[gix] / HH * p.gy[gjy] / HH * p.gz[k] / HH;
+                        ft = ft - p.tp * OCV * 0.5 * p.ramp * rinv * c_gx / HH * c_gy / HH * uload(p.gz, k) / HH;
                     else
-                        ft = ft - p.tp * OCV * rinv * p.gx[gix] / HH * p.gy[gjy] / HH * p.gz[k] / HH;
+                        ft = ft - p.tp * OCV * rinv * c_gx / HH * c_gy / HH * uload(p.gz, k) / HH;
                 }
             }
             // LSHR: no viscous or compressional heating (sub:1903-2101)
