@@ -54,6 +54,17 @@ module rast3dmhd_c
       real(c_double), intent(in) :: ro0(*), tt0(*)
     end function
 
+    integer(c_int) function mhd_rebase(h) bind(c, name='mhd_rebase')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+
+    integer(c_int) function mhd_set_rebase_interval(h, nsteps) bind(c, name='mhd_set_rebase_interval')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+    end function
+
     integer(c_int) function mhd_upload(h, ru, rv, rw, ro, tt, bx, by, bz) bind(c, name='mhd_upload')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

@@ -100,6 +100,7 @@ C     ZEE)
       CHARACTER(LEN=16) :: PREC
       CHARACTER(LEN=40) :: WHAT
       INTEGER :: PLEN, PST, NF, IERR, IR, PY, PZ, J, K, JG, KG
+      INTEGER :: NRB, IOS
       INTEGER :: NSEND, NMV
       LOGICAL :: DUMP, FL
 C
@@ -344,6 +345,19 @@ C     RAST3DMHD_PRECISION=mixed selects the mixed-precision form
             RETURN
          ENDIF
       ENDIF
+C     RAST3DMHD_REBASE=N re-centres the mixed form's background on the
+C     horizontal means every N steps (mhd_set_rebase_interval)
+      NRB=0
+      CALL GET_ENVIRONMENT_VARIABLE('RAST3DMHD_REBASE',PREC,PLEN,PST)
+      IF ((PST.EQ.0).AND.(PLEN.GT.0)) THEN
+         READ(PREC(1:PLEN),*,IOSTAT=IOS) NRB
+         IF ((IOS.NE.0).OR.(NRB.LT.0)) THEN
+            WRITE(6,*)'STEP: RAST3DMHD_REBASE is a number of steps'
+            IRC=1
+            WHAT='STEP: RAST3DMHD_REBASE'
+            RETURN
+         ENDIF
+      ENDIF
       D%HX=HX
       D%H2X=H2X
       D%HY=HY
@@ -441,6 +455,11 @@ C        background of the mixed form: horizontal means of rho and T
          WHAT='STEP: mhd_set_background'
          IRC=MHD_SET_BACKGROUND(H,RO0,TT0)
          IF (IRC.NE.0) GOTO 9
+         IF (NRB.GT.0) THEN
+            WHAT='STEP: mhd_set_rebase_interval'
+            IRC=MHD_SET_REBASE_INTERVAL(H,NRB)
+            IF (IRC.NE.0) GOTO 9
+         ENDIF
       ENDIF
       WHAT='STEP: mhd_upload'
       IF (NPE.EQ.1) THEN

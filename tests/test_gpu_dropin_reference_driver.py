@@ -36,10 +36,12 @@ def exe_of(name):
     return exe
 
 
-def run(name, cwd, ntotal, nstart=0, precision=None, timeout=120):
+def run(name, cwd, ntotal, nstart=0, precision=None, timeout=120, rebase=None):
     env = dict(os.environ, RAST_NTOTAL=str(ntotal), RAST_NSTEP0="1", RAST_NSTART=str(nstart))
     if precision:
         env["RAST3DMHD_PRECISION"] = precision
+    if rebase is not None:
+        env["RAST3DMHD_REBASE"] = rebase
     _, c = deck(name)
     npe = int(c["params"]["NPEY"]) * int(c["params"]["NPEZ"])
     return subprocess.run([MPIEXEC, "-launcher", "fork", "-n", str(npe), exe_of(name)], cwd=cwd, env=env,
@@ -52,6 +54,9 @@ def test_dropin_driver_stops_every_rank_on_a_shim_error(tmp_path):
     so this runs without a GPU."""
     r = run("drv12", tmp_path, 3, precision="bogus", timeout=60)
     assert "RAST3DMHD_PRECISION is fp64 or mixed" in r.stdout, r.stdout + r.stderr
+    assert not os.path.exists(tmp_path / "rast3d.dat0.0000")
+    r = run("drv12", tmp_path, 3, precision="mixed", rebase="often", timeout=60)
+    assert "RAST3DMHD_REBASE is a number of steps" in r.stdout, r.stdout + r.stderr
     assert not os.path.exists(tmp_path / "rast3d.dat0.0000")
 
 
@@ -115,10 +120,13 @@ def test_reference_driver_with_gpu_step_stops_the_nonfinite_mhd_restart(tmp_path
 
 
 @pytest.mark.gpu
-def test_reference_driver_with_mixed_gpu_step(tmp_path):
-    print("case reference driver + GPU STEP, mixed form", flush=True)
+@pytest.mark.parametrize("rebase", [None, "1"])
+def test_reference_driver_with_mixed_gpu_step(tmp_path, rebase):
+    """The mixed form under the reference driver, with and without the
+    background re-centred after every step (RAST3DMHD_REBASE)."""
+    print("case reference driver + GPU STEP, mixed form, rebase", rebase, flush=True)
     _, c = deck("drv12")
-    r = run("drv12", tmp_path, int(c["nsteps"]), precision="mixed")
+    r = run("drv12", tmp_path, int(c["nsteps"]), precision="mixed", rebase=rebase)
     assert r.returncode == 0, r.stdout + r.stderr
     lay = layout(c)
     for k in (1, 2, 3):
