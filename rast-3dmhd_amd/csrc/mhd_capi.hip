@@ -356,11 +356,13 @@ static int rebase(mhd_t *h) {
     const Params &p = h->p;
     const int nz = p.npz;
     const size_t nzp = (size_t)nz + 2;
-    if (!h->bgd2) {
-        hipError_t e = hipMalloc(&h->bgd2, BG_DBL * nzp * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&h->bgf2, BG_FLT * nzp * sizeof(float));
-        if (e == hipSuccess) e = hipMalloc(&h->bgp2, nzp * sizeof(PlaneM));
-        if (e == hipSuccess) e = hipMalloc(&h->prof, 3 * (size_t)nz * sizeof(double));
+    if (nz < 3) return fail(h, MHD_ERR_ARG, "mhd_rebase: npz=%d < 3", nz);
+    {
+        hipError_t e = hipSuccess;
+        if (!h->bgd2) e = hipMalloc(&h->bgd2, BG_DBL * nzp * sizeof(double));
+        if (e == hipSuccess && !h->bgf2) e = hipMalloc(&h->bgf2, BG_FLT * nzp * sizeof(float));
+        if (e == hipSuccess && !h->bgp2) e = hipMalloc(&h->bgp2, nzp * sizeof(PlaneM));
+        if (e == hipSuccess && !h->prof) e = hipMalloc(&h->prof, 3 * (size_t)nz * sizeof(double));
         if (e != hipSuccess) return fail(h, MHD_ERR_HIP, "mhd_rebase: allocation: %s", hipGetErrorString(e));
     }
     std::vector<double> prof(3 * (size_t)nz);
@@ -410,6 +412,7 @@ extern "C" int mhd_set_rebase_interval(mhd_t *h, int nsteps) {
     if (!h) return fail(nullptr, MHD_ERR_ARG, "mhd_set_rebase_interval: null handle");
     if (nsteps < 0) return fail(h, MHD_ERR_ARG, "mhd_set_rebase_interval: nsteps=%d < 0", nsteps);
     if (nsteps && !h->mixed) return fail(h, MHD_ERR_STATE, "mhd_set_rebase_interval: only for MHD_MIXED handles");
+    if (nsteps && h->p.npz < 3) return fail(h, MHD_ERR_ARG, "mhd_set_rebase_interval: npz=%d < 3", h->p.npz);
     h->rebase_every = nsteps;
     h->since_rebase = 0;
     return MHD_OK;
