@@ -4,9 +4,25 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 struct mhd_net;
 
 namespace rmhd {
+
+// A kernel's dynamic-LDS limit, set once per device: function attributes
+// belong to the device current at the call, so a process that drives
+// several GPUs sets them on each.  done: one bit per device index (< 64).
+inline hipError_t lds_limit_once(std::atomic<unsigned long long> &done, const void *fn, int bytes) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const unsigned long long bit = (dev >= 0 && dev < 64) ? 1ull << dev : 0ull;
+    if (bit && (done.load(std::memory_order_acquire) & bit)) return hipSuccess;
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e == hipSuccess && bit) done.fetch_or(bit, std::memory_order_acq_rel);
+    return e;
+}
 
 // Field order in every device register (q, Z, F).
 enum Field { F_RU = 0, F_RV, F_RW, F_RO, F_TT, F_BX, F_BY, F_BZ, NFIELD };

@@ -928,12 +928,10 @@ template <int TY, bool MAG, int MODE, bool OPT>
 static hipError_t launch_stage_o(const Params &p, const Metrics &m, const double *qin, double *qout, double *z,
                                  double *fout, DevScalars *sc, hipStream_t s) {
     constexpr int lds = stage_lds_bytes<TY, MAG>();
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<TY, MAG, MODE, OPT>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    static std::atomic<unsigned long long> attr{0};
+    {
+        hipError_t e = lds_limit_once(attr, reinterpret_cast<const void *>(&k_stage<TY, MAG, MODE, OPT>), lds);
         if (e != hipSuccess) return e;
-        attr = true;
     }
     int nch;
     const int kc = chunk_for(p, TY, &nch);
@@ -979,12 +977,10 @@ template <int TY, bool MAG, int MODE>
 static hipError_t launch_batch_t(const BoxDev *boxes, const StageItem *items, int nitems, int grid, int a,
                                  hipStream_t s) {
     constexpr int lds = stage_lds_bytes<TY, MAG>();
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_batch<TY, MAG, MODE>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    static std::atomic<unsigned long long> attr{0};
+    {
+        hipError_t e = lds_limit_once(attr, reinterpret_cast<const void *>(&k_stage_batch<TY, MAG, MODE>), lds);
         if (e != hipSuccess) return e;
-        attr = true;
     }
     hipLaunchKernelGGL((k_stage_batch<TY, MAG, MODE>), dim3(grid), dim3(TX * TY), lds, s, boxes, items, nitems, a);
     return hipGetLastError();

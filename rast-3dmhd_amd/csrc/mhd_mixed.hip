@@ -822,12 +822,10 @@ template <int TY, bool MAG, int MODE>
 static hipError_t launch_m(const Params &p, const Metrics &m, const Background &bg, const float *hin, float *hout,
                            float *dr, unsigned *rr, DevScalars *sc, hipStream_t s) {
     constexpr int lds = mixed_lds_bytes<TY, MAG, MODE>();
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_m<TY, MAG, MODE>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    static std::atomic<unsigned long long> attr{0};
+    {
+        hipError_t e = lds_limit_once(attr, reinterpret_cast<const void *>(&k_stage_m<TY, MAG, MODE>), lds);
         if (e != hipSuccess) return e;
-        attr = true;
     }
     int nch;
     const int kc = chunk_m(p, TY, &nch);
@@ -910,12 +908,11 @@ template <int TY, bool MAG, int MODE, int TXW>
 static hipError_t launch_mb(const BoxDev *boxes, const StageItem *items, int nitems, int grid, int a,
                             hipStream_t s) {
     constexpr int lds = mixed_lds_bytes<TY, MAG, MODE, TXW>();
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage_m_batch<TY, MAG, MODE, TXW>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    static std::atomic<unsigned long long> attr{0};
+    {
+        hipError_t e =
+            lds_limit_once(attr, reinterpret_cast<const void *>(&k_stage_m_batch<TY, MAG, MODE, TXW>), lds);
         if (e != hipSuccess) return e;
-        attr = true;
     }
     hipLaunchKernelGGL((k_stage_m_batch<TY, MAG, MODE, TXW>), dim3(grid), dim3(TXW * TY), lds, s, boxes, items,
                        nitems, a);

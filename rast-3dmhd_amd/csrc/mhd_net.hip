@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "../../include/rast3dmhd_closure.h"
+#include "mhd_kernels.h"
 
 namespace {
 
@@ -53,7 +54,6 @@ constexpr int XS = LW + 7;                // dW: LDS row stride of X (73, odd)
 
 template <int CIN>
 constexpr int cstride() { return ((CIN / 8) % 2 == 0) ? CIN + 8 : CIN; }
-
 
 __device__ __forceinline__ int wrapn(int x, int n) {
     int r = x % n;
@@ -83,7 +83,8 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // EPI 4: out16 = gelu(acc + bias) (inference: the pre-activation, which only
 // the backward pass reads, is not stored).
 template <int CIN, int NT, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_conv(const bf16 *__restrict__ x, const bf16 *__restrict__ wt, int kp,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_conv(const bf16 *__restrict__ x, const bf16 *__restrict__ wt, int kp,
                                               const float *__restrict__ bias, const float *__restrict__ zprev,
                                               bf16 *__restrict__ out16, float *__restrict__ out32,
                                               float *__restrict__ zout, int cout, int ostride, int nx, int ny,
@@ -162,8 +163,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // operand fragments of K block kb (A: 4 M tiles of this wavefront's
-        // row, B: the NT output-channel tiles)
+        // the A fragments of K block kb: 4 M tiles of this wavefront's row
         auto frag = [&](int kb, bf16x8 *a) __attribute__((always_inline)) {
             const int k0 = kb * 32 + lg * 8;
             if (k0 < KTOT) {
@@ -539,28 +539,28 @@ static hipError_t conv(hipStream_t s, const bf16 *x, const bf16 *w, int kp, cons
     if (kp != 32 * ((27 * CIN + 31) / 32)) return hipErrorInvalidValue;  // the kernel's K blocks
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if ((long long)nx * ny * CIN > 0x7fffffffLL) return hipErrorInvalidValue;  // in-plane offsets are 32-bit
-    static int attr = 0;
-    if (lds > attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv<CIN, NT, EPI>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        if (e != hipSuccess) return e;
-        attr = lds;
-    }
+    static std::atomic<unsigned long long> attr{0};
+    hipError_t e = rmhd::lds_limit_once(attr, reinterpret_cast<const void *>(&k_conv<CIN, NT, EPI>), lds);
+    if (e != hipSuccess) return e;
     // as many workgroups as fit the CUs at once (LDS and registers): each
     // streams a longer z chunk, which amortises its prologue (three planes +
-    // the weights)
-    static int resident = 0;
-    if (!resident) {
-        int per_cu = 0, ncu = 0, dev = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&k_conv<CIN, NT, EPI>), 256, lds);
-        if (e == hipSuccess) e = hipGetDevice(&dev);
+    // the weights); per device
+    static std::atomic<int> resident[64];
+    int dev = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    int res = resident[dev].load(std::memory_order_relaxed);
+    if (!res) {
+        int per_cu = 0, ncu = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(&k_conv<CIN, NT, EPI>),
+                                                         256, lds);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return e;
-        resident = std::max(1, per_cu) * std::max(1, ncu);
+        res = std::max(1, per_cu) * std::max(1, ncu);
+        resident[dev].store(res, std::memory_order_relaxed);
     }
     int nb;
-    const int kc = kchunk_for(nx, ny, nz, resident, &nb);
+    const int kc = kchunk_for(nx, ny, nz, res, &nb);
     hipLaunchKernelGGL((k_conv<CIN, NT, EPI>), dim3(nb), dim3(256), lds, s, x, w, kp, bias, zprev, out16, out32, zout,
                        cout, ostride, nx, ny, nz, kc);
     return hipGetLastError();
