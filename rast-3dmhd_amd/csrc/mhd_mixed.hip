@@ -336,115 +336,123 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 }
             }
 
-            // continuity
-            rhs_t fr = (CUR(0, -1) - CUR(0, 1)) * hxdx - (CUR(1, SW) - CUR(1, -SW)) * hydy;
-            if (top) fr -= (rhs_t(4) * rw_p - rwk2) * hzdz;
-            else if (bot) fr -= (rwk2 - rhs_t(4) * rw_m) * hzdz;
-            else fr -= (rw_p - rw_m) * hzdz;
-            // buoyancy and pressure gradient of the perturbation
-            rhs_t fw = GRAV * rh0 - (pp - pm) * hzdz;
-            rhs_t fv = (pym - pyp) * hydy;
-            rhs_t fu = (pxm - pxp) * hxdx;
-            // momentum advection
-            fu -= (CUR(0, 1) * uxp - CUR(0, -1) * uxm) * hxdx + (CUR(0, SW) * vyp - CUR(0, -SW) * vym) * hydy
-                  + (ru_p * wp - ru_m * wm) * hzdz;
-            fv -= (CUR(1, 1) * uxp - CUR(1, -1) * uxm) * hxdx + (CUR(1, SW) * vyp - CUR(1, -SW) * vym) * hydy
-                  + (rv_p * wp - rv_m * wm) * hzdz;
-            fw -= (CUR(2, 1) * uxp - CUR(2, -1) * uxm) * hxdx + (CUR(2, SW) * vyp - CUR(2, -SW) * vym) * hydy
-                  + (rw_p * wp - rw_m * wm) * hzdz;
-            // temperature advection, the background gradient included
-            const rhs_t txp = RNG(k, 3, 1), txm = RNG(k, 3, -1), typ = RNG(k, 3, SW), tym = RNG(k, 3, -SW);
-            const rhs_t dtz = (th_p - th_m) * hzdz;
-            rhs_t ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + PLB(dt0dz32, k)));
-            // conduction of T': (1/rho) c [kappa lap T' + kappa' dT'/dz]; plus the
-            // change of the background's conduction with 1/rho: -rho'/(rho rho0) S0
+            // The FP32 right-hand side is evaluated with its multiply-adds
+            // fused (one rounding instead of two).  The mixed form is judged
+            // by its error against the FP64 form, not bit for bit; fusion is
+            // decided per source expression, so every kernel that inlines
+            // this body computes the same values.
+            rhs_t fr, fw, fv, fu, ft, w1 = rhs_t(0), w2 = rhs_t(0), w3 = rhs_t(0);
             {
-                const rhs_t lap = (txp - txm) * hxd2x + (txp - rhs_t(2) * th0 + txm) * h2xdx2
-                                  + (typ - tym) * hyd2y + (typ - rhs_t(2) * th0 + tym) * h2ydy2
-                                  + (th_p - th_m) * hzd2z + (th_p - rhs_t(2) * th0 + th_m) * h2zdz2;
-                ft += rinv * PLP(t_cond, k) * (rk * lap + dkap * dtz) - rh0 * rinv * PLB(rinv0_32, k) * PLB(s0_32, k);
-            }
-            // viscous Laplacians
-            fu += ORE * (C43 * ((uxp - uxm) * hxd2x + (uxp - rhs_t(2) * u0 + uxm) * h2xdx2)
-                         + (uyp - uym) * hyd2y + (uyp - rhs_t(2) * u0 + uym) * h2ydy2
-                         + (up - um) * hzd2z + (up - rhs_t(2) * u0 + um) * h2zdz2);
-            fv += ORE * ((vxp - vxm) * hxd2x + (vxp - rhs_t(2) * v0 + vxm) * h2xdx2
-                         + C43 * ((vyp - vym) * hyd2y + (vyp - rhs_t(2) * v0 + vym) * h2ydy2)
-                         + (vp - vm) * hzd2z + (vp - rhs_t(2) * v0 + vm) * h2zdz2);
-            fw += ORE * ((wxp - wxm) * hxd2x + (wxp - rhs_t(2) * w0 + wxm) * h2xdx2
-                         + (wyp - wym) * hyd2y + (wyp - rhs_t(2) * w0 + wym) * h2ydy2
-                         + C43 * ((wp - wm) * hzd2z + (wp - rhs_t(2) * w0 + wm) * h2zdz2));
-            // heating and grad div u cross terms
-            const rhs_t tv = PLP(t_visc, k), tv43 = PLP(t_visc43, k), tcr = PLP(t_cross, k);
-            const rhs_t dxu = (uxp - uxm) * hxdx, dxv = (vxp - vxm) * hxdx, dxw = (wxp - wxm) * hxdx;
-            const rhs_t dyu = (uyp - uym) * hydy, dyv = (vyp - vym) * hydy, dyw = (wyp - wym) * hydy;
-            const rhs_t dzu = (up - um) * hzdz, dzv = (vp - vm) * hzdz, dzw = (wp - wm) * hzdz;
-            ft += (C43 * dxu * dxu + dxv * dxv + dxw * dxw + dyu * dyu + rhs_t(2) * dyu * dxv) * rinv * tv
-                  + (dyv * dyv + dzw * dzw - dxu * dzw - dxu * dyv - dyv * dzw) * rinv * tv43
-                  + (dzu * dzu + dzv * dzv + dyw * dyw + rhs_t(2) * dzv * dyw + rhs_t(2) * dzu * dxw) * rinv * tv
-                  - OCV * (dxu + dyv + dzw) * tf0;
-            fu += tcr * (((RNG(k, 1, SW + 1) - RNG(k, 1, SW - 1)) - (RNG(k, 1, -SW + 1) - RNG(k, 1, -SW - 1))) * hxdx * hydy
-                         + ((RNG(k + 1, 2, 1) - RNG(k - 1, 2, 1)) - (RNG(k + 1, 2, -1) - RNG(k - 1, 2, -1))) * hzdz * hxdx);
-            fv += tcr * (((RNG(k, 0, SW + 1) - RNG(k, 0, SW - 1)) - (RNG(k, 0, -SW + 1) - RNG(k, 0, -SW - 1))) * hxdx * hydy
-                         + ((RNG(k + 1, 2, SW) - RNG(k - 1, 2, SW)) - (RNG(k + 1, 2, -SW) - RNG(k - 1, 2, -SW))) * hzdz * hydy);
-            fw += tcr * (((RNG(k + 1, 0, 1) - RNG(k - 1, 0, 1)) - (RNG(k + 1, 0, -1) - RNG(k - 1, 0, -1))) * hzdz * hxdx
-                         + ((RNG(k + 1, 1, SW) - RNG(k - 1, 1, SW)) - (RNG(k + 1, 1, -SW) - RNG(k - 1, 1, -SW))) * hzdz * hydy);
-            // Coriolis
-            if (p.lrot) {
-                const rhs_t omx = PLP(omx, k), omz = PLP(omz, k);
-                fu += omz * q0[1];
-                fv += -omz * q0[0] + omx * q0[2];
-                fw -= omx * q0[1];
-            }
-            // MHD
-            rhs_t w1 = rhs_t(0), w2 = rhs_t(0), w3 = rhs_t(0);
-            if (MAG) {
-                const rhs_t bx0 = q0[5], by0 = q0[6], bz0 = q0[7];
-                const rhs_t bxm = rm_.v[5], bym = rm_.v[6], bzm = rm_.v[7];
-                const rhs_t bxp = NXT(5), byp = NXT(6), bzp = NXT(7);
-                const rhs_t bxxp = CUR(5, 1), bxxm = CUR(5, -1), bxyp = CUR(5, SW), bxym = CUR(5, -SW);
-                const rhs_t byxp = CUR(6, 1), byxm = CUR(6, -1), byyp = CUR(6, SW), byym = CUR(6, -SW);
-                const rhs_t bzxp = CUR(7, 1), bzxm = CUR(7, -1), bzyp = CUR(7, SW), bzym = CUR(7, -SW);
-                rhs_t dzw1 = dzw;
-                if (top) dzw1 = (rhs_t(4) * wp - wk2) * hzdz;
-                else if (bot) dzw1 = (wk2 - rhs_t(4) * wm) * hzdz;
-                const rhs_t dxbx = (bxxp - bxxm) * hxdx, dxby = (byxp - byxm) * hxdx, dxbz = (bzxp - bzxm) * hxdx;
-                const rhs_t dybx = (bxyp - bxym) * hydy, dyby = (byyp - byym) * hydy, dybz = (bzyp - bzym) * hydy;
-                const rhs_t dzbx = (bxp - bxm) * hzdz, dzby = (byp - bym) * hzdz, dzbz = (bzp - bzm) * hzdz;
-                // induction (B.grad) u - B div u - (u.grad) B
-                w1 = by0 * dyu - bx0 * dyv + bz0 * dzu - bx0 * dzw1 - u0 * dxbx - v0 * dybx - w0 * dzbx;
-                w2 = bx0 * dxv - by0 * dxu + bz0 * dzv - by0 * dzw1 - u0 * dxby - v0 * dyby - w0 * dzby;
-                w3 = bx0 * dxw - bz0 * dxu + by0 * dyw - bz0 * dyv - u0 * dxbz - v0 * dybz - w0 * dzbz;
-                // Lorentz force
-                fu += OBETA * (by0 * (dybx - dxby) - bz0 * dxbz + bz0 * dzbx);
-                fv += OBETA * (bx0 * (dxby - dybx) + bz0 * (dzby - dybz));
-                fw += OBETA * (bx0 * dxbz - bx0 * dzbx - by0 * (dzby - dybz));
-                // Ohmic heating |J|^2 / rho
-                ft += ((dybx - dxby) * (dybx - dxby) + dxbz * dxbz + dzbx * dzbx - rhs_t(2) * dzbx * dxbz
-                       + (dzby - dybz) * (dzby - dybz)) * rinv * PLP(t_ohm, k);
-                // resistive diffusion, one-sided in z on the boundary planes for BX, BY
-                rhs_t d2zbx, d2zby;
-                if (top) {
-                    d2zbx = (-rhs_t(3) * bx0 + rhs_t(4) * bxp - bxk2) * hzd2z + (rhs_t(2) * bx0 - rhs_t(5) * bxp + rhs_t(4) * bxk2 - bxk3) * h2zdz2;
-                    d2zby = (-rhs_t(3) * by0 + rhs_t(4) * byp - byk2) * hzd2z + (rhs_t(2) * by0 - rhs_t(5) * byp + rhs_t(4) * byk2 - byk3) * h2zdz2;
-                } else if (bot) {
-                    d2zbx = (rhs_t(3) * bx0 - rhs_t(4) * bxm + bxk2) * hzd2z + (rhs_t(2) * bx0 - rhs_t(5) * bxm + rhs_t(4) * bxk2 - bxk3) * h2zdz2;
-                    d2zby = (rhs_t(3) * by0 - rhs_t(4) * bym + byk2) * hzd2z + (rhs_t(2) * by0 - rhs_t(5) * bym + rhs_t(4) * byk2 - byk3) * h2zdz2;
-                } else {
-                    d2zbx = (bxp - bxm) * hzd2z + (bxp - rhs_t(2) * bx0 + bxm) * h2zdz2;
-                    d2zby = (byp - bym) * hzd2z + (byp - rhs_t(2) * by0 + bym) * h2zdz2;
+#pragma clang fp contract(on)
+                // continuity
+                fr = (CUR(0, -1) - CUR(0, 1)) * hxdx - (CUR(1, SW) - CUR(1, -SW)) * hydy;
+                if (top) fr -= (rhs_t(4) * rw_p - rwk2) * hzdz;
+                else if (bot) fr -= (rwk2 - rhs_t(4) * rw_m) * hzdz;
+                else fr -= (rw_p - rw_m) * hzdz;
+                // buoyancy and pressure gradient of the perturbation
+                fw = GRAV * rh0 - (pp - pm) * hzdz;
+                fv = (pym - pyp) * hydy;
+                fu = (pxm - pxp) * hxdx;
+                // momentum advection
+                fu -= (CUR(0, 1) * uxp - CUR(0, -1) * uxm) * hxdx + (CUR(0, SW) * vyp - CUR(0, -SW) * vym) * hydy
+                      + (ru_p * wp - ru_m * wm) * hzdz;
+                fv -= (CUR(1, 1) * uxp - CUR(1, -1) * uxm) * hxdx + (CUR(1, SW) * vyp - CUR(1, -SW) * vym) * hydy
+                      + (rv_p * wp - rv_m * wm) * hzdz;
+                fw -= (CUR(2, 1) * uxp - CUR(2, -1) * uxm) * hxdx + (CUR(2, SW) * vyp - CUR(2, -SW) * vym) * hydy
+                      + (rw_p * wp - rw_m * wm) * hzdz;
+                // temperature advection, the background gradient included
+                const rhs_t txp = RNG(k, 3, 1), txm = RNG(k, 3, -1), typ = RNG(k, 3, SW), tym = RNG(k, 3, -SW);
+                const rhs_t dtz = (th_p - th_m) * hzdz;
+                ft = -(u0 * (txp - txm) * hxdx + v0 * (typ - tym) * hydy + w0 * (dtz + PLB(dt0dz32, k)));
+                // conduction of T': (1/rho) c [kappa lap T' + kappa' dT'/dz]; plus the
+                // change of the background's conduction with 1/rho: -rho'/(rho rho0) S0
+                {
+                    const rhs_t lap = (txp - txm) * hxd2x + (txp - rhs_t(2) * th0 + txm) * h2xdx2
+                                      + (typ - tym) * hyd2y + (typ - rhs_t(2) * th0 + tym) * h2ydy2
+                                      + (th_p - th_m) * hzd2z + (th_p - rhs_t(2) * th0 + th_m) * h2zdz2;
+                    ft += rinv * PLP(t_cond, k) * (rk * lap + dkap * dtz) - rh0 * rinv * PLB(rinv0_32, k) * PLB(s0_32, k);
                 }
-                const rhs_t d2zbz = (bzp - bzm) * hzd2z + (bzp - rhs_t(2) * bz0 + bzm) * h2zdz2;
-                w1 += ORM * ((bxxp - bxxm) * hxd2x + (bxxp - rhs_t(2) * bx0 + bxxm) * h2xdx2
-                             + (bxyp - bxym) * hyd2y + (bxyp - rhs_t(2) * bx0 + bxym) * h2ydy2 + d2zbx);
-                w2 += ORM * ((byxp - byxm) * hxd2x + (byxp - rhs_t(2) * by0 + byxm) * h2xdx2
-                             + (byyp - byym) * hyd2y + (byyp - rhs_t(2) * by0 + byym) * h2ydy2 + d2zby);
-                w3 += ORM * ((bzxp - bzxm) * hxd2x + (bzxp - rhs_t(2) * bz0 + bzxm) * h2xdx2
-                             + (bzyp - bzym) * hyd2y + (bzyp - rhs_t(2) * bz0 + bzym) * h2ydy2 + d2zbz);
-            }
+                // viscous Laplacians
+                fu += ORE * (C43 * ((uxp - uxm) * hxd2x + (uxp - rhs_t(2) * u0 + uxm) * h2xdx2)
+                             + (uyp - uym) * hyd2y + (uyp - rhs_t(2) * u0 + uym) * h2ydy2
+                             + (up - um) * hzd2z + (up - rhs_t(2) * u0 + um) * h2zdz2);
+                fv += ORE * ((vxp - vxm) * hxd2x + (vxp - rhs_t(2) * v0 + vxm) * h2xdx2
+                             + C43 * ((vyp - vym) * hyd2y + (vyp - rhs_t(2) * v0 + vym) * h2ydy2)
+                             + (vp - vm) * hzd2z + (vp - rhs_t(2) * v0 + vm) * h2zdz2);
+                fw += ORE * ((wxp - wxm) * hxd2x + (wxp - rhs_t(2) * w0 + wxm) * h2xdx2
+                             + (wyp - wym) * hyd2y + (wyp - rhs_t(2) * w0 + wym) * h2ydy2
+                             + C43 * ((wp - wm) * hzd2z + (wp - rhs_t(2) * w0 + wm) * h2zdz2));
+                // heating and grad div u cross terms
+                const rhs_t tv = PLP(t_visc, k), tv43 = PLP(t_visc43, k), tcr = PLP(t_cross, k);
+                const rhs_t dxu = (uxp - uxm) * hxdx, dxv = (vxp - vxm) * hxdx, dxw = (wxp - wxm) * hxdx;
+                const rhs_t dyu = (uyp - uym) * hydy, dyv = (vyp - vym) * hydy, dyw = (wyp - wym) * hydy;
+                const rhs_t dzu = (up - um) * hzdz, dzv = (vp - vm) * hzdz, dzw = (wp - wm) * hzdz;
+                ft += (C43 * dxu * dxu + dxv * dxv + dxw * dxw + dyu * dyu + rhs_t(2) * dyu * dxv) * rinv * tv
+                      + (dyv * dyv + dzw * dzw - dxu * dzw - dxu * dyv - dyv * dzw) * rinv * tv43
+                      + (dzu * dzu + dzv * dzv + dyw * dyw + rhs_t(2) * dzv * dyw + rhs_t(2) * dzu * dxw) * rinv * tv
+                      - OCV * (dxu + dyv + dzw) * tf0;
+                fu += tcr * (((RNG(k, 1, SW + 1) - RNG(k, 1, SW - 1)) - (RNG(k, 1, -SW + 1) - RNG(k, 1, -SW - 1))) * hxdx * hydy
+                             + ((RNG(k + 1, 2, 1) - RNG(k - 1, 2, 1)) - (RNG(k + 1, 2, -1) - RNG(k - 1, 2, -1))) * hzdz * hxdx);
+                fv += tcr * (((RNG(k, 0, SW + 1) - RNG(k, 0, SW - 1)) - (RNG(k, 0, -SW + 1) - RNG(k, 0, -SW - 1))) * hxdx * hydy
+                             + ((RNG(k + 1, 2, SW) - RNG(k - 1, 2, SW)) - (RNG(k + 1, 2, -SW) - RNG(k - 1, 2, -SW))) * hzdz * hydy);
+                fw += tcr * (((RNG(k + 1, 0, 1) - RNG(k - 1, 0, 1)) - (RNG(k + 1, 0, -1) - RNG(k - 1, 0, -1))) * hzdz * hxdx
+                             + ((RNG(k + 1, 1, SW) - RNG(k - 1, 1, SW)) - (RNG(k + 1, 1, -SW) - RNG(k - 1, 1, -SW))) * hzdz * hydy);
+                // Coriolis
+                if (p.lrot) {
+                    const rhs_t omx = PLP(omx, k), omz = PLP(omz, k);
+                    fu += omz * q0[1];
+                    fv += -omz * q0[0] + omx * q0[2];
+                    fw -= omx * q0[1];
+                }
+                // MHD
+                if (MAG) {
+                    const rhs_t bx0 = q0[5], by0 = q0[6], bz0 = q0[7];
+                    const rhs_t bxm = rm_.v[5], bym = rm_.v[6], bzm = rm_.v[7];
+                    const rhs_t bxp = NXT(5), byp = NXT(6), bzp = NXT(7);
+                    const rhs_t bxxp = CUR(5, 1), bxxm = CUR(5, -1), bxyp = CUR(5, SW), bxym = CUR(5, -SW);
+                    const rhs_t byxp = CUR(6, 1), byxm = CUR(6, -1), byyp = CUR(6, SW), byym = CUR(6, -SW);
+                    const rhs_t bzxp = CUR(7, 1), bzxm = CUR(7, -1), bzyp = CUR(7, SW), bzym = CUR(7, -SW);
+                    rhs_t dzw1 = dzw;
+                    if (top) dzw1 = (rhs_t(4) * wp - wk2) * hzdz;
+                    else if (bot) dzw1 = (wk2 - rhs_t(4) * wm) * hzdz;
+                    const rhs_t dxbx = (bxxp - bxxm) * hxdx, dxby = (byxp - byxm) * hxdx, dxbz = (bzxp - bzxm) * hxdx;
+                    const rhs_t dybx = (bxyp - bxym) * hydy, dyby = (byyp - byym) * hydy, dybz = (bzyp - bzym) * hydy;
+                    const rhs_t dzbx = (bxp - bxm) * hzdz, dzby = (byp - bym) * hzdz, dzbz = (bzp - bzm) * hzdz;
+                    // induction (B.grad) u - B div u - (u.grad) B
+                    w1 = by0 * dyu - bx0 * dyv + bz0 * dzu - bx0 * dzw1 - u0 * dxbx - v0 * dybx - w0 * dzbx;
+                    w2 = bx0 * dxv - by0 * dxu + bz0 * dzv - by0 * dzw1 - u0 * dxby - v0 * dyby - w0 * dzby;
+                    w3 = bx0 * dxw - bz0 * dxu + by0 * dyw - bz0 * dyv - u0 * dxbz - v0 * dybz - w0 * dzbz;
+                    // Lorentz force
+                    fu += OBETA * (by0 * (dybx - dxby) - bz0 * dxbz + bz0 * dzbx);
+                    fv += OBETA * (bx0 * (dxby - dybx) + bz0 * (dzby - dybz));
+                    fw += OBETA * (bx0 * dxbz - bx0 * dzbx - by0 * (dzby - dybz));
+                    // Ohmic heating |J|^2 / rho
+                    ft += ((dybx - dxby) * (dybx - dxby) + dxbz * dxbz + dzbx * dzbx - rhs_t(2) * dzbx * dxbz
+                           + (dzby - dybz) * (dzby - dybz)) * rinv * PLP(t_ohm, k);
+                    // resistive diffusion, one-sided in z on the boundary planes for BX, BY
+                    rhs_t d2zbx, d2zby;
+                    if (top) {
+                        d2zbx = (-rhs_t(3) * bx0 + rhs_t(4) * bxp - bxk2) * hzd2z + (rhs_t(2) * bx0 - rhs_t(5) * bxp + rhs_t(4) * bxk2 - bxk3) * h2zdz2;
+                        d2zby = (-rhs_t(3) * by0 + rhs_t(4) * byp - byk2) * hzd2z + (rhs_t(2) * by0 - rhs_t(5) * byp + rhs_t(4) * byk2 - byk3) * h2zdz2;
+                    } else if (bot) {
+                        d2zbx = (rhs_t(3) * bx0 - rhs_t(4) * bxm + bxk2) * hzd2z + (rhs_t(2) * bx0 - rhs_t(5) * bxm + rhs_t(4) * bxk2 - bxk3) * h2zdz2;
+                        d2zby = (rhs_t(3) * by0 - rhs_t(4) * bym + byk2) * hzd2z + (rhs_t(2) * by0 - rhs_t(5) * bym + rhs_t(4) * byk2 - byk3) * h2zdz2;
+                    } else {
+                        d2zbx = (bxp - bxm) * hzd2z + (bxp - rhs_t(2) * bx0 + bxm) * h2zdz2;
+                        d2zby = (byp - bym) * hzd2z + (byp - rhs_t(2) * by0 + bym) * h2zdz2;
+                    }
+                    const rhs_t d2zbz = (bzp - bzm) * hzd2z + (bzp - rhs_t(2) * bz0 + bzm) * h2zdz2;
+                    w1 += ORM * ((bxxp - bxxm) * hxd2x + (bxxp - rhs_t(2) * bx0 + bxxm) * h2xdx2
+                                 + (bxyp - bxym) * hyd2y + (bxyp - rhs_t(2) * bx0 + bxym) * h2ydy2 + d2zbx);
+                    w2 += ORM * ((byxp - byxm) * hxd2x + (byxp - rhs_t(2) * by0 + byxm) * h2xdx2
+                                 + (byyp - byym) * hyd2y + (byyp - rhs_t(2) * by0 + byym) * h2ydy2 + d2zby);
+                    w3 += ORM * ((bzxp - bzxm) * hxd2x + (bzxp - rhs_t(2) * bz0 + bzxm) * h2xdx2
+                                 + (bzyp - bzym) * hyd2y + (bzyp - rhs_t(2) * bz0 + bzym) * h2ydy2 + d2zbz);
+                }
 #undef CUR
 #undef NXT
 #undef RNG
+            }
             // FP64 accumulation: F = F_bg(z) + F', X = value + D + gamma dt F
             // (the temperature slot's value is T'(rho', P')); store the new
             // FP32 values and carry the residual (+ zeta dt F) in D.
