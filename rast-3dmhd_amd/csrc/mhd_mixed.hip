@@ -455,7 +455,9 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             }
             // FP64 accumulation: F = F_bg(z) + F', X = value + D + gamma dt F
             // (the temperature slot's value is T'(rho', P')); store the new
-            // FP32 values and carry the residual (+ zeta dt F) in D.
+            // FP32 values and carry the residual (+ zeta dt F) in D.  The
+            // products dt F are fused into their sums; the residual itself
+            // (X minus its stored representation) is exact either way.
             double Fd[8] = {fu, fv, fw, fr, ft, w1, w2, w3};
             Fd[2] += PLB(fwbg, k);
             Fd[4] += PLB(ftbg, k);
@@ -476,6 +478,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
             float hn[8];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
+#pragma clang fp contract(on)
                 const double v = f == 4 ? (double)th0 : (double)h0[f];
                 // (read only once this field's right-hand side exists)
                 const float dkf = MODE == 1 ? r_half(after(rk_[f >> 1], Fd[f]), f) : after(dk_[f], Fd[f]);
@@ -489,6 +492,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 const double r = xq[f] - (f == 4 ? (double)trep(PLB(ro32, k), PLB(tt32, k), hn[3], hn[4]) : (double)hn[f]);
                 __builtin_nontemporal_store(hn[f], &hout[f * fs + o]);
                 if (MODE < 3) {
+#pragma clang fp contract(on)
                     __builtin_nontemporal_store((float)(r + coefz * Fd[f]), &dr[f * fs + o]);
                 } else {
                     const unsigned b = to_bf16((float)r);
