@@ -220,8 +220,9 @@ typedef struct mhd_dev_metrics {
  *    after the stream has finished the step.
  * The two RK registers, the Z register and the reduction scalars live in
  * one static device array of at most 64 MiB per GPU (held until the process
- * ends), so 3 * nf * 8 * NPX*NPY*(NPZ+2) bytes (nf = 8 with LMAG, else 5)
- * must fit in 64 MiB - 256 B (e.g. 64x64x32 MHD); larger boxes are rejected
+ * ends), so 3 * nf * 8 * (NPX*NPY*(NPZ+2) + 1184) bytes (nf = 8 with LMAG,
+ * else 5; 1184 values of padding per field) must fit in 64 MiB - 256 B
+ * (e.g. 64x64x32 MHD); larger boxes are rejected
  * with MHD_ERR_ARG.  Every call on a GPU shares that GPU's array: calls that
  * overlap on one GPU (other threads or streams) race.  Errors are reported
  * through mhd_last_error(NULL). */

@@ -82,7 +82,7 @@ inline Params params_from_desc(const mhd_desc *d) {
     p.tile_y = 4;
     if (const char *ty = getenv("RAST3DMHD_TILE_Y")) p.tile_y = atoi(ty) == 8 ? 8 : 4;
     p.plane = (long long)d->npx * d->npy;
-    p.fsize = p.plane * (d->npz + 2);
+    p.fsize = p.plane * (d->npz + 2) + FIELD_PAD;
     p.hx = d->hx; p.h2x = d->h2x; p.hy = d->hy; p.h2y = d->h2y; p.hz = d->hz; p.h2z = d->h2z;
     p.c13 = d->c13; p.c23 = d->c23; p.c43 = d->c43;
     p.gam1 = d->gam1; p.gam2 = d->gam2; p.gam3 = d->gam3; p.zeta1 = d->zeta1; p.zeta2 = d->zeta2;

@@ -27,6 +27,13 @@ inline hipError_t lds_limit_once(std::atomic<unsigned long long> &done, const vo
 // Field order in every device register (q, Z, F).
 enum Field { F_RU = 0, F_RV, F_RW, F_RO, F_TT, F_BX, F_BY, F_BZ, NFIELD };
 
+// Elements between the end of one field's planes and the start of the
+// next: 37 lines of 128 B in FP32 (74 in FP64), so that the fields of one
+// point do not all fall into the same L2 set, as they do when the field
+// stride is a multiple of the plane size of a power-of-two box (the stage
+// kernels then re-read halo rows that such conflicts had evicted).
+constexpr long long FIELD_PAD = 1184;
+
 // Scalars of the step, all as the driver computed them, plus products the
 // reference forms once per FLUXES call (e.g. TMP=OCV*ORE), computed on the
 // host with the same association.
@@ -35,7 +42,7 @@ struct Params {
     int lmag, lrot, izc, itc, ibc;
     int tile_y;                 // rows per stage-kernel tile (4 or 8)
     long long plane;            // npx*npy
-    long long fsize;            // plane*(npz+2): one field incl. z ghost planes
+    long long fsize;            // field stride: plane*(npz+2) values incl. z ghost planes, + FIELD_PAD
     double hx, h2x, hy, h2y, hz, h2z, c13, c23, c43;
     double gam1, gam2, gam3, zeta1, zeta2;
     double cv, ocv, ore, re, repr, theta, grav, sf, gamma;

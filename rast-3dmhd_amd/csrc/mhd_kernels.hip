@@ -825,7 +825,7 @@ __global__ void k_cfl_final_batch(const BoxDev *__restrict__ boxes, int nbox, in
 
 // padded Fortran (npx+2, npy+2, npz+2) -> resident field (x/y ghosts dropped)
 __global__ void k_pack(const Params p, const double *__restrict__ padded, double *__restrict__ field) {
-    const long long n = p.fsize;
+    const long long n = p.plane * (p.npz + 2);  // the field's values, not its padded stride
     const int nxp = p.npx + 2, nyp = p.npy + 2;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long long)gridDim.x * blockDim.x) {
@@ -1053,7 +1053,7 @@ hipError_t launch_cfl_final(const Params &p, DevScalars *sc, bool clock, hipStre
 }
 
 hipError_t launch_pack(const Params &p, const double *padded, double *field, hipStream_t s) {
-    long long nb = (p.fsize + 255) / 256;
+    long long nb = (p.plane * (p.npz + 2) + 255) / 256;
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(k_pack, dim3((unsigned)nb), dim3(256), 0, s, p, padded, field);
     return hipGetLastError();

@@ -695,7 +695,7 @@ __global__ __launch_bounds__(256) void k_cfl_m(const Params p, const Metrics m, 
 // the already packed rho' of the same register.
 __global__ void k_pack_m(const Params p, const double *__restrict__ padded, int field, const Background bg,
                          float *__restrict__ hreg, unsigned *__restrict__ rreg) {
-    const long long n = p.fsize, fs = p.fsize;
+    const long long n = p.plane * (p.npz + 2), fs = p.fsize;  // n: the field's values
     const int nxp = p.npx + 2, nyp = p.npy + 2;
     float *const h = hreg + field * fs;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
@@ -892,7 +892,7 @@ hipError_t launch_cfl_mixed(const Params &p, const Metrics &m, const Background 
 
 hipError_t launch_pack_mixed(const Params &p, const double *padded, int field, const Background &bg, float *hreg,
                              unsigned *rreg, hipStream_t s) {
-    long long nb = (p.fsize + 255) / 256;
+    long long nb = (p.plane * (p.npz + 2) + 255) / 256;
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(k_pack_m, dim3((unsigned)nb), dim3(256), 0, s, p, padded, field, bg, hreg, rreg);
     return hipGetLastError();

@@ -7,7 +7,8 @@
 // of NOALLOC_BYTES per GPU:
 //
 //   [0, 256)             DevScalars (reduction keys, DT, clock)
-//   [256, 256 + 3*R)     q[0], q[1], Z; R = nf * fsize doubles each
+//   [256, 256 + 3*R)     q[0], q[1], Z; R = nf * fsize doubles each (fsize
+//                        the field stride of Params, padding included)
 //
 // On the host the name `scratch` stands for no device address: its address
 // on each device is looked up once per device with hipGetSymbolAddress and
