@@ -27,6 +27,10 @@
 // compensated pair and the storage rounding does not accumulate from step
 // to step.  Stage 1 reads R, stages 1-2 write D, stage 3 writes R.
 // The CFL reduction, the boundary planes and the background are FP64.
+// The file is compiled without FMA contraction; the FP32 right-hand side
+// and the FP64 accumulation of the stage body opt in (#pragma clang fp
+// contract), everything that packs, unpacks or reconstructs stored values
+// (trep, pprime, the residuals) rounds each operation separately.
 //
 // Traffic per zone (8 fields): stage 1 reads h (4 B) and R (2 B) and writes
 // h and D (4 + 4 B); stage 2 reads and writes h and D; stage 3 reads h and D
