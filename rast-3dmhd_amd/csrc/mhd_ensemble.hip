@@ -635,13 +635,14 @@ extern "C" int mhd_ens_autotune(mhd_ens_t *e, int nsteps, char *report, int repo
     const int t0x = e->tile_x, t0y = e->tile_y, t0k = e->kdepth, t0g = e->grid;
     const int txs[3] = {64, 64, 32}, tys[3] = {4, 8, 8};  // 32x8 in the mixed form only
     const int nshape = e->mixed ? 3 : 2;
-    const int kds[4] = {8, 16, 32, 0};
+    constexpr int NKD = 7;
+    const int kds[NKD] = {8, 12, 16, 20, 24, 32, 0};
     const int per_cu[4] = {0, 1, 2, 4};  // 0: one workgroup per item
     double best = 1e300;
     int btx = t0x, bty = t0y, bkd = t0k, bgr = t0g;
     std::string rep = "[";
     for (int a = 0; a < nshape && r == hipSuccess && !rc; ++a)
-        for (int k = 0; k < 4 && r == hipSuccess && !rc; ++k)
+        for (int k = 0; k < NKD && r == hipSuccess && !rc; ++k)
             for (int g = 0; g < 4 && r == hipSuccess && !rc; ++g) {
                 const int grid = per_cu[g] * e->cus;
                 rc = mhd_ens_set_tile(e, txs[a], tys[a], kds[k], grid);

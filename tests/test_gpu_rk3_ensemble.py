@@ -101,7 +101,7 @@ def test_ensemble_autotune_keeps_state():
         ens.steps(2)
         report = ens.autotune(nsteps=2)
         say("autotune", ens.tiling(), min(r["ms_step"] for r in report))
-        assert len(report) == 32
+        assert len(report) == 56  # 2 shapes x 7 depths x 4 grids
         best = min(report, key=lambda r: r["ms_step"])
         assert ens.tiling() == (best["tile_y"], best["kdepth"], best["grid"])
         ens.steps(3)
@@ -109,7 +109,7 @@ def test_ensemble_autotune_keeps_state():
 
 
 def test_ensemble_autotune_mixed_tile_shapes():
-    """The mixed form also tries 32x8 tiles: 3 shapes x 4 depths x 4 grids."""
+    """The mixed form also tries 32x8 tiles: 3 shapes x 7 depths x 4 grids."""
     say("case ensemble autotune mixed")
     bx = boxes(SPECS[:2] + [ODD])
     refs = [alone(m, s, "mixed", 3) for m, s in bx]
@@ -118,8 +118,8 @@ def test_ensemble_autotune_mixed_tile_shapes():
             ens.upload(b, s)
         ens.steps(1)
         report = ens.autotune(nsteps=1)
-        assert len(report) == 48
-        assert sum(r["tile_x"] == 32 for r in report) == 16
+        assert len(report) == 84
+        assert sum(r["tile_x"] == 32 for r in report) == 28
         best = min(report, key=lambda r: r["ms_step"])
         say("autotune", ens.tile(), best["ms_step"])
         assert ens.tile() == (best["tile_x"], best["tile_y"], best["kdepth"], best["grid"])
