@@ -71,12 +71,13 @@ __device__ __forceinline__ float trep(float r0, float t0, float rh, float pp) {
 }
 
 // BF16 bit patterns of the resting residual R: round to nearest even
-// (NaN stays NaN), and back (exact).
-__host__ __device__ __forceinline__ unsigned short to_bf16(float x) {
-    unsigned u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
+// (v_cvt_pk_bf16_f32; NaN stays NaN), and back (exact).
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned short to_bf16(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
+// Two residuals in one 32-bit word: lo in the low half, hi in the high half.
+__device__ __forceinline__ unsigned pack_bf16x2(float lo, float hi) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2_t){lo, hi}, bf16x2_t));
 }
 __host__ __device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 

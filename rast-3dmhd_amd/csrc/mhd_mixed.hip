@@ -490,7 +490,7 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
                 if (f != 4) hn[f] = (float)xq[f];
             }
             hn[4] = (float)pprime(r0d, t0d, xq[3], xq[4]);
-            unsigned rw = 0;
+            float rs[8];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const double r = xq[f] - (f == 4 ? (double)trep(PLB(ro32, k), PLB(tt32, k), hn[3], hn[4]) : (double)hn[f]);
@@ -499,10 +499,15 @@ __device__ __forceinline__ void stage_tile_m(const Params &p, const Metrics &m, 
 #pragma clang fp contract(on)
                     __builtin_nontemporal_store((float)(r + coefz * Fd[f]), &dr[f * fs + o]);
                 } else {
-                    const unsigned b = to_bf16((float)r);
-                    rw = (f & 1) ? (rw | (b << 16)) : b;
-                    if ((f & 1) || f + 1 == NF) __builtin_nontemporal_store(rw, &rr[(f >> 1) * fs + o]);
+                    rs[f] = (float)r;
                 }
+            }
+            if (MODE == 3) {
+                // the residuals as BF16 pairs (an odd last field with a zero high half)
+#pragma unroll
+                for (int w = 0; w < (NF + 1) / 2; ++w)
+                    __builtin_nontemporal_store(pack_bf16x2(rs[2 * w], 2 * w + 1 < NF ? rs[2 * w + 1] : 0.0f),
+                                                &rr[w * fs + o]);
             }
             if (MODE == 3 && !top && !bot) {
                 double full[8];
