@@ -73,7 +73,9 @@ int mhd_ens_get_tile(const mhd_ens_t *e, int *tile_x, int *tile_y, int *kdepth, 
  * launches), keeps the fastest, and restores the state and clocks of every
  * box to what they were before the call.  report (may be NULL) receives a
  * JSON list of {tile_x, tile_y, kdepth, grid, ms_step} for the candidates
- * (tile shapes as in mhd_ens_set_tile). */
+ * (tile shapes as in mhd_ens_set_tile; 7 chunk depths x 4 grids per shape,
+ * about 80 bytes each: 8 KiB holds the list, a shorter report_len cuts it
+ * at report_len - 1 bytes). */
 int mhd_ens_autotune(mhd_ens_t *e, int nsteps, char *report, int report_len);
 
 /* Mean milliseconds per ensemble step over n steps (HIP events on the
